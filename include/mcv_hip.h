@@ -1,0 +1,124 @@
+/*
+ * mcv_hip.h -- C-ABI of the MI355X (gfx950) MCV shallow-water time step.
+ *
+ * Drop-in for the reference's time-integration seam: a host program that
+ * owns the namelist, mesh, initial state and output (the reference's
+ * MCV_SW.F90) hands the mesh and the initial state over once, then advances
+ * the state on the GPU and reads it back at output times.
+ *
+ * Arrays use the reference layout: column-major (i, j, panel), the halo
+ * included, i and j in [1-H, nPV+H] with H = dof-1 and nPV = n*H+1, panel in
+ * [1, 6] (stat_mod.F90:7-45).  2x2 matrices are (2, 2, i, j, panel)
+ * column-major (mesh_mod.F90:20-23).  The library copies what it needs during
+ * each call and never keeps a host pointer after the call returns.
+ *
+ * Every entry point returns 0 on success or a negative MCV_E* code; the
+ * message of the last failure on the calling thread is mcv_hip_last_error().
+ * No entry point calls exit() or abort().  A handle is used by one host
+ * thread at a time.
+ */
+#ifndef MCV_HIP_H
+#define MCV_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCV_ABI_VERSION 1
+
+enum {
+  MCV_OK = 0,
+  MCV_EINVAL = -1,     /* bad argument or configuration */
+  MCV_ENOMEM = -2,     /* device or host allocation failed */
+  MCV_EDEVICE = -3,    /* HIP runtime error */
+  MCV_ENONFINITE = -4, /* NaN/Inf found in the FP64 mass/energy integrals */
+  MCV_ESTATE = -5      /* call not valid in the handle's current state */
+};
+
+/* time_settings%integral_scheme (parameters_mod.F90:208-214) */
+enum { MCV_RK3_TVD = 3, MCV_RK4 = 4 };
+
+/* arithmetic of the step */
+enum {
+  MCV_FP64 = 0, /* the reference's FP64 arithmetic, bit for bit */
+  MCV_FP32C = 1 /* FP32 bulk with FP64 metrics, ghost weights and integrals */
+};
+
+typedef struct mcv_config {
+  int32_t abi_version; /* MCV_ABI_VERSION */
+  int32_t dof;         /* 3 (MCV3) or 4 (MCV4): constants_mod.F90:4-10 */
+  int32_t n;           /* cells per panel edge, Nx = 90/dx: parameters_mod.F90:148 */
+  int32_t scheme;      /* MCV_RK4 or MCV_RK3_TVD */
+  int32_t precision;   /* MCV_FP64 or MCV_FP32C */
+  int32_t device;      /* HIP device ordinal */
+  double dt;           /* time step, s (parameters_mod.F90:11) */
+  double dx;           /* element width, radians (parameters_mod.F90:200) */
+} mcv_config;
+
+/* Host-computed geometry (mesh_mod.F90:18-40, ghost_mod.f90:11-18).
+ * matrixIG must be symmetric bit for bit (projection_mod.F90:186 makes it so). */
+typedef struct mcv_mesh {
+  const double *sqrtG;      /* (nh, nh, 6) */
+  const double *matrixG;    /* (2, 2, nh, nh, 6) */
+  const double *matrixIG;   /* (2, 2, nh, nh, 6) */
+  const double *matrixA;    /* (2, 2, nh, nh, 6) */
+  const double *matrixIA;   /* (2, 2, nh, nh, 6) */
+  const double *f;          /* Coriolis parameter, (nh, nh, 6) */
+  const double *phi_s;      /* surface geopotential, (nh, nh, 6) */
+  const double *ghost_coef; /* (nPV, H): offset of the ghost point in its cell */
+  const int32_t *ghost_iref;/* (nPV, H): 1-based cell holding the ghost point */
+} mcv_mesh;
+
+/* Prognostic state with its diagnosed companions (stat_mod.F90:7-20).
+ * phiG must equal phi*sqrtG bit for bit (test_case_mod.F90:30). */
+typedef struct mcv_state {
+  double *phi, *phiG, *u, *v, *contraU, *contraV; /* each (nh, nh, 6) */
+} mcv_state;
+
+typedef struct mcv_handle mcv_handle;
+
+/* Replaces the device-side half of MCV_SW.F90:36-41 (initStat, initTend and
+ * the state handoff of test_case_mod.F90:30-43): uploads the mesh and the
+ * full initial state, halo included.  The first stage of the first step uses
+ * the given contraU/contraV and halo, as the reference does. */
+int mcv_hip_init(const mcv_config *cfg, const mcv_mesh *mesh,
+                 const mcv_state *initial, mcv_handle **out);
+
+/* Replaces `call RK4(stat(new), stat(old))` / `call RK3_TVD(...)` followed by
+ * switch_stat (MCV_SW.F90:58-59,76; time_scheme_mod.F90:13-142): enqueues
+ * nsteps full time steps on the handle's stream and returns. */
+int mcv_hip_step(mcv_handle *h, int32_t nsteps);
+
+/* Blocks until queued steps finish, then copies the state back in the
+ * reference layout (any field pointer may be NULL) and returns the FP64
+ * integrals of diag_mod.F90:10-67 (mass, energy may be NULL).  Values at the
+ * four halo corners of each panel are unspecified (the reference never reads
+ * them).  Returns MCV_ENONFINITE if either integral is NaN or Inf. */
+int mcv_hip_sync(mcv_handle *h, mcv_state *out, double *mass, double *energy);
+
+/* Sphere (zonal, meridional) winds of the current state on every point, as
+ * the reference's history output computes them (spatial_operators_mod.F90:
+ * 391-405, called at MCV_SW.F90:65).  Blocks like mcv_hip_sync. */
+int mcv_hip_sphere_winds(mcv_handle *h, double *zonal, double *meridional);
+
+/* The spatial operator alone (spatial_operators_mod.F90:12-133) applied to
+ * the given full state: tendencies of phiG, u, v on (nPV, nPV, 6).  Does not
+ * change the handle's state.  Blocks. */
+int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG,
+                     double *t_u, double *t_v);
+
+/* Number of completed time steps since mcv_hip_init. */
+int mcv_hip_steps_done(const mcv_handle *h, int64_t *steps);
+
+/* Releases the handle and its device memory.  NULL is accepted. */
+int mcv_hip_free(mcv_handle *h);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mcv_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,45 @@
+"""In-tree build of the product libraries.
+
+  libmcv_hip.so   HIP kernels + C-ABI for gfx950 (hipcc)
+
+Every HIP source is compiled with -ffp-contract=off (the FP64 path must round
+exactly where the reference does) and with packed-FP32 instructions disabled.
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HIP_LIB = os.path.join(HERE, "libmcv_hip.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def build_hip(verbose=False):
+    src = os.path.join(CSRC, "mcv_hip.hip")
+    tmp = HIP_LIB + ".tmp"
+    cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp, src]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + p.stderr[-6000:])
+    if verbose and p.stderr.strip():
+        print(p.stderr)
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+def build_all(verbose=False):
+    return [build_hip(verbose)]
+
+
+if __name__ == "__main__":
+    for path in build_all(verbose=True):
+        print("built", path)

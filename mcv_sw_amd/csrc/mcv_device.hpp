@@ -1,0 +1,189 @@
+// Device-side building blocks of the MCV step: MCV reconstruction formulas,
+// the Riemann flux, the cubed-sphere connectivity tables and the work
+// descriptors passed to the kernels.
+//
+// FP64 arithmetic here keeps the reference's operand order, and the library
+// is compiled with -ffp-contract=off, so the FP64 path rounds exactly where
+// the reference Fortran rounds (division and sqrt are IEEE-correct on gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcv {
+
+// ---------------------------------------------------------------------------
+// Per-resolution geometry and per-instance state, as the kernels see them.
+// ---------------------------------------------------------------------------
+
+// Static mesh data of one resolution on the device.  Arrays are in the
+// reference layout (nh*nh points per panel, i fastest, halo included).  The
+// metric terms sqrtG and IG are identical on all six panels of the reference
+// mesh; metric_pstride is 0 then (one copy serves every panel) and nh*nh
+// otherwise.  The same holds for phi_s with phis_pstride.
+struct MeshDev {
+  int dof, n, npv, nh;
+  double dx;                 // element width, radians
+  long long metric_pstride;  // panel stride of sqrtG, ig11, ig12, ig22
+  long long phis_pstride;    // panel stride of phi_s
+  const double *sqrtG, *ig11, *ig12, *ig22;
+  const double *f;           // per panel (f differs between panels 5 and 6)
+  const double *phi_s;
+  const double *p2sp;        // (A*IG) per point, 4 doubles (m11, m21, m12, m22)
+  const double *sp2p;        // (G*IA) per point, 4 doubles
+  const double *gcoef;       // (H, nPV): ghost offsets (ghost_mod.f90:11-18)
+  const int32_t *giref;      // (H, nPV): 1-based cells
+};
+
+// One RK stage of one instance: where the stage reads and writes.
+struct StageArgs {
+  const double *in_phi, *in_u, *in_v;  // stage state Y_s (halo filled)
+  const double *b_phi, *b_u, *b_v;     // RK base state Y_n
+  double *acc_phiG, *acc_u, *acc_v;    // RK4 tendency accumulator
+  double *o_phi, *o_u, *o_v;           // stage output (interior, raw)
+  double *edge_raw;                    // raw copy of the 4 edge lines, see edge_slot()
+  const double *cu0, *cv0;             // host contravariant winds (first stage only)
+  double c1;                           // stage increment coefficient (x dt)
+};
+
+struct BoundaryArgs {
+  double *phi, *u, *v;                 // level to finish (halo + edges)
+  const double *edge_raw;
+};
+
+// Stage kinds of time_scheme_mod.F90:13-123.
+enum StageKind : int {
+  kTend = 0,   // spatial operator only (tendency seam)
+  kRK4S1 = 1,  // Y2 = Yn + dt/2 k1;      acc = k1
+  kRK4S2 = 2,  // Y3 = Yn + dt/2 k2;      acc += 2 k2
+  kRK4S3 = 3,  // Y4 = Yn + dt k3;        acc += 2 k3
+  kRK4S4 = 4,  // Yn+1 = Yn + dt (acc + k4)/6
+  kRK3S1 = 5,  // Y1 = Yn + dt k1
+  kRK3S2 = 6,  // Y2 = 3/4 Yn + 1/4 Y1 + 1/4 dt k2
+  kRK3S3 = 7,  // Yn+1 = Yn/3 + 2/3 Y2 + 2/3 dt k3
+};
+
+// Slot of the raw edge-line copy: panel p (1..6), side (0 W, 1 E, 2 S, 3 N),
+// field (0 phi, 1 u, 2 v), along-edge index s (1..nPV).
+__host__ __device__ inline long long edge_slot(int npv, int p, int side, int field, int s) {
+  return ((long long)((p - 1) * 4 + side) * 3 + field) * npv + (s - 1);
+}
+
+// Linear index of point (i, j) of panel p (1-based, halo-relative i, j).
+__host__ __device__ inline long long pidx(int nh, int H, int p, int i, int j) {
+  return ((long long)(p - 1) * nh + (j + H - 1)) * (long long)nh + (i + H - 1);
+}
+
+// ---------------------------------------------------------------------------
+// Cubed-sphere connectivity (ghost_mod.f90:88-152, spatial_operators_mod.F90:
+// 298-386).  Sides: 0 W (i = 1), 1 E (i = nPV), 2 S (j = 1), 3 N (j = nPV).
+// ---------------------------------------------------------------------------
+
+// Source of the halo of (panel, side) at depth k, along index s: interior
+// point of panel q on the line at normal index (far ? nPV-k : k+1), which is
+// a column (i fixed) if col, traversed reversed (nPV+1-s) if rev.
+struct HaloSrc { int8_t q, col, far, rev; };
+// A cube edge shared by (pa, sa) and (pb, sb); b traversed reversed if rev.
+struct EdgePair { int8_t pa, sa, pb, sb, rev; };
+// A panel corner: panel, i-corner, j-corner (0 -> index 1, 1 -> nPV).
+struct VtxPt { int8_t p, ci, cj; };
+
+extern __constant__ HaloSrc kHalo[6][4];
+extern __constant__ EdgePair kEdges[12];
+extern __constant__ int8_t kFirstEdge[6][4];
+extern __constant__ VtxPt kVerts[8][3];
+
+// ---------------------------------------------------------------------------
+// MCV formulas (FP64, reference evaluation order)
+// ---------------------------------------------------------------------------
+
+// polyfitL: one-sided derivative at the cell's left end (spatial_operators_mod.F90:204-218)
+template <int DOF>
+__device__ __forceinline__ double fit_left(const double *pv, double dh) {
+  if constexpr (DOF == 3) {
+    return -((pv[2] - 4.0 * pv[1] + 3.0 * pv[0]) / dh);
+  } else {
+    return (-(11.0 * pv[0]) + 18.0 * pv[1] - 9.0 * pv[2] + 2.0 * pv[3]) / (2.0 * dh);
+  }
+}
+
+// polyfitR: one-sided derivative at the cell's right end (spatial_operators_mod.F90:220-234)
+template <int DOF>
+__device__ __forceinline__ double fit_right(const double *pv, double dh) {
+  if constexpr (DOF == 3) {
+    return (pv[0] - 4.0 * pv[1] + 3.0 * pv[2]) / dh;
+  } else {
+    return (11.0 * pv[3] - 18.0 * pv[2] + 9.0 * pv[1] - 2.0 * pv[0]) / (2.0 * dh);
+  }
+}
+
+// Rusanov flux on derivatives (spatial_operators_mod.F90:236-246)
+__device__ __forceinline__ double riemann(double fxL, double fxR, double qxL, double qxR,
+                                          double lam) {
+  return 0.5 * ((fxL + fxR) - lam * (qxR - qxL));
+}
+
+// Largest gravity-wave speed along one direction (spatial_operators_mod.F90:248-280)
+__device__ __forceinline__ double wave_speed(double c, double phi, double gii) {
+  double l1 = c - sqrt(gii * phi), l2 = c + sqrt(gii * phi);
+  double a1 = fabs(l1), a2 = fabs(l2);
+  return a1 > a2 ? a1 : a2;
+}
+
+// sign of the contravariant wind with 0 -> +1 (spatial_operators_mod.F90:83-85)
+__device__ __forceinline__ double upwind_sign(double c) {
+  if (c == 0.0) c = 1.0;
+  return c / fabs(c);
+}
+
+// Interior-point derivative of a cell from its point values and the two
+// cell-end derivatives (spatial_operators_mod.F90:178-200).  pos in [1, H-1].
+template <int DOF>
+__device__ __forceinline__ double cell_interior(const double *f, double dL, double dR,
+                                                int pos, double dx) {
+  if constexpr (DOF == 3) {
+    double via = (f[2] - f[0]) / dx;
+    return 1.5 * via - 0.25 * (dL + dR);
+  } else {
+    double via = (f[3] - f[0]) / dx;
+    double fxx = 4.5 * (f[0] - f[1] - f[2] + f[3]) / (dx * dx);
+    if (pos == 1) return 4.0 / 3.0 * via - (4.0 * dL + 5.0 * dR) / 27.0 - 4.0 / 27.0 * dx * fxx;
+    return 4.0 / 3.0 * via - (5.0 * dL + 4.0 * dR) / 27.0 + 4.0 / 27.0 * dx * fxx;
+  }
+}
+
+// Degree DOF-1 Lagrange interpolation of a cell at offset c (ghost_mod.f90:183-227)
+template <int DOF>
+__device__ __forceinline__ double ghost_interp(const double *q, double c, double dx) {
+  if constexpr (DOF == 3) {
+    return q[0] - (3.0 * q[0] - 4.0 * q[1] + q[2]) * c / dx +
+           2.0 * (q[0] - 2.0 * q[1] + q[2]) * (c * c) / (dx * dx);
+  } else {
+    double dx3 = dx * dx * dx, c3 = c * c * c;
+    return (2.0 * dx3 * q[0] +
+            (dx * dx) * (-(11.0 * q[0]) + 18.0 * q[1] - 9.0 * q[2] + 2.0 * q[3]) * c +
+            9.0 * dx * (2.0 * q[0] - 5.0 * q[1] + 4.0 * q[2] - q[3]) * (c * c) +
+            9.0 * (-q[0] + 3.0 * q[1] - 3.0 * q[2] + q[3]) * c3) /
+           (2.0 * dx3);
+  }
+}
+
+// 2x2 apply with M = (m11, m21, m12, m22) (projection_mod.F90:60-96)
+__device__ __forceinline__ void apply2(const double *M, double a, double b, double &o1,
+                                       double &o2) {
+  o1 = M[0] * a + M[2] * b;
+  o2 = M[1] * a + M[3] * b;
+}
+
+// blockIdx -> logical work index so that the blocks one XCD runs are
+// consecutive in the logical order.  Blocks are dealt round-robin to the 8
+// XCDs of an MI355X, each with its own L2; consecutive logical indices touch
+// neighbouring tiles and the same metric tile on the six panels.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  constexpr int kXcd = 8;
+  int q = nb / kXcd, r = nb % kXcd;
+  int x = b % kXcd, slot = b / kXcd;
+  return x * q + (x < r ? x : r) + slot;
+}
+
+}  // namespace mcv

@@ -1,0 +1,505 @@
+// C-ABI of the MCV step on MI355X (include/mcv_hip.h): handle lifetime,
+// host-side preparation of the static mesh data, launch sequences for the
+// RK3_TVD and RK4 schemes, read-back and FP64 integrals.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mcv_hip.h"
+#include "mcv_device.hpp"
+#include "mcv_kernels_fp64.hpp"
+
+namespace mcv {
+
+// Connectivity tables (ghost_mod.f90:104-146, spatial_operators_mod.F90:298-386).
+__constant__ HaloSrc kHalo[6][4] = {
+    /* P1 */ {{4, 1, 1, 0}, {2, 1, 0, 0}, {6, 0, 1, 0}, {5, 0, 0, 0}},
+    /* P2 */ {{1, 1, 1, 0}, {3, 1, 0, 0}, {6, 1, 1, 1}, {5, 1, 1, 0}},
+    /* P3 */ {{2, 1, 1, 0}, {4, 1, 0, 0}, {6, 0, 0, 1}, {5, 0, 1, 1}},
+    /* P4 */ {{3, 1, 1, 0}, {1, 1, 0, 0}, {6, 1, 0, 0}, {5, 1, 0, 1}},
+    /* P5 */ {{4, 0, 1, 1}, {2, 0, 1, 0}, {1, 0, 1, 0}, {3, 0, 1, 1}},
+    /* P6 */ {{4, 0, 0, 0}, {2, 0, 0, 1}, {3, 0, 0, 1}, {1, 0, 0, 0}},
+};
+__constant__ EdgePair kEdges[12] = {
+    {1, 0, 4, 1, 0}, {2, 0, 1, 1, 0}, {3, 0, 2, 1, 0}, {4, 0, 3, 1, 0},
+    {1, 3, 5, 2, 0}, {1, 2, 6, 3, 0}, {2, 3, 5, 1, 0}, {2, 2, 6, 1, 1},
+    {3, 3, 5, 3, 1}, {3, 2, 6, 2, 1}, {4, 3, 5, 0, 1}, {4, 2, 6, 0, 0},
+};
+// index into kEdges of the edge whose first panel side is (panel, side), or -1
+__constant__ int8_t kFirstEdge[6][4] = {
+    {0, -1, 5, 4}, {1, -1, 7, 6}, {2, -1, 9, 8}, {3, -1, 11, 10}, {-1, -1, -1, -1},
+    {-1, -1, -1, -1},
+};
+__constant__ VtxPt kVerts[8][3] = {
+    {{1, 0, 1}, {4, 1, 1}, {5, 0, 0}}, {{1, 1, 1}, {2, 0, 1}, {5, 1, 0}},
+    {{1, 0, 0}, {4, 1, 0}, {6, 0, 1}}, {{1, 1, 0}, {2, 0, 0}, {6, 1, 1}},
+    {{3, 0, 1}, {2, 1, 1}, {5, 1, 1}}, {{3, 1, 1}, {4, 0, 1}, {5, 0, 1}},
+    {{2, 1, 0}, {3, 0, 0}, {6, 1, 0}}, {{3, 1, 0}, {4, 0, 0}, {6, 0, 0}},
+};
+
+}  // namespace mcv
+
+using namespace mcv;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(MCV_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+struct DevBuf {
+  double *p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct mcv_handle {
+  mcv_config cfg{};
+  int H = 0, npv = 0, nh = 0;
+  long long np = 0;  // points per field
+  hipStream_t stream = nullptr;
+  MeshDev mesh{};
+  std::vector<void *> allocs;  // every device allocation, freed together
+  double *lv[3][3] = {};       // level (Yn, S1, S2) x field (phi, u, v)
+  double *acc[3] = {};
+  double *edge_raw = nullptr;
+  double *cu0 = nullptr, *cv0 = nullptr;  // host contra winds until the first stage
+  bool first_pending = true;
+  double *partials = nullptr;
+  int diag_blocks = 0;
+  double *diag_out = nullptr;
+  double *pack[6] = {};
+  int64_t steps = 0;
+  int tiles_x = 0;
+};
+
+namespace {
+
+int dev_alloc(mcv_handle *h, double **out, size_t count) {
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(double) + 256);
+  if (e != hipSuccess)
+    return fail(MCV_ENOMEM, "hipMalloc(%zu doubles) failed: %s", count, hipGetErrorString(e));
+  h->allocs.push_back(p);
+  *out = static_cast<double *>(p);
+  return MCV_OK;
+}
+
+int upload(mcv_handle *h, double **dst, const double *src, size_t count) {
+  int rc = dev_alloc(h, dst, count);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(*dst, src, count * sizeof(double), hipMemcpyHostToDevice));
+  return MCV_OK;
+}
+
+bool bits_equal(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
+
+// True when the six panels of a (nh*nh*6) array hold identical bits.
+bool panel_invariant(const double *a, long long per, int width = 1) {
+  for (int p = 1; p < 6; ++p)
+    if (std::memcmp(a, a + p * per * width, sizeof(double) * per * width) != 0) return false;
+  return true;
+}
+
+void destroy(mcv_handle *h) {
+  if (!h) return;
+  if (h->stream) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipStreamDestroy(h->stream);
+  }
+  for (void *p : h->allocs) (void)hipFree(p);
+  delete h;
+}
+
+int validate(const mcv_config *c) {
+  if (!c) return fail(MCV_EINVAL, "cfg is NULL");
+  if (c->abi_version != MCV_ABI_VERSION)
+    return fail(MCV_EINVAL, "abi_version %d, library is %d", c->abi_version, MCV_ABI_VERSION);
+  if (c->dof != 3 && c->dof != 4) return fail(MCV_EINVAL, "dof must be 3 or 4, got %d", c->dof);
+  if (c->n < 2 || c->n > 8192) return fail(MCV_EINVAL, "n must be in [2, 8192], got %d", c->n);
+  if (c->scheme != MCV_RK4 && c->scheme != MCV_RK3_TVD)
+    return fail(MCV_EINVAL, "Unknown integral scheme %d, select MCV_RK3_TVD or MCV_RK4",
+                c->scheme);
+  if (c->precision != MCV_FP64)
+    return fail(MCV_EINVAL, "precision %d is not available through this entry point",
+                c->precision);
+  if (!(c->dt > 0.0) || !std::isfinite(c->dt)) return fail(MCV_EINVAL, "dt must be > 0");
+  if (!(c->dx > 0.0) || !std::isfinite(c->dx)) return fail(MCV_EINVAL, "dx must be > 0");
+  return MCV_OK;
+}
+
+template <int DOF>
+int launch_stage(mcv_handle *h, int kind, const StageArgs &a) {
+  constexpr int C = Tile<DOF>::C;
+  const int tx = (h->cfg.n + C - 1) / C;
+  dim3 grid(tx * tx * 6), block(kStageThreads);
+  switch (kind) {
+#define MCV_CASE(K)                                                                  \
+  case K:                                                                            \
+    hipLaunchKernelGGL((stage_kernel<DOF, K>), grid, block, 0, h->stream, h->mesh, a, tx); \
+    break;
+    MCV_CASE(kTend)
+    MCV_CASE(kRK4S1)
+    MCV_CASE(kRK4S2)
+    MCV_CASE(kRK4S3)
+    MCV_CASE(kRK4S4)
+    MCV_CASE(kRK3S1)
+    MCV_CASE(kRK3S2)
+    MCV_CASE(kRK3S3)
+#undef MCV_CASE
+    default:
+      return fail(MCV_EINVAL, "bad stage kind %d", kind);
+  }
+  HIP_TRY(hipGetLastError());
+  return MCV_OK;
+}
+
+int stage(mcv_handle *h, int kind, const StageArgs &a) {
+  return h->cfg.dof == 3 ? launch_stage<3>(h, kind, a) : launch_stage<4>(h, kind, a);
+}
+
+int boundary(mcv_handle *h, int level) {
+  BoundaryArgs b{h->lv[level][0], h->lv[level][1], h->lv[level][2], h->edge_raw};
+  const int threads = 6 * 4 * h->npv + 8;
+  dim3 grid((threads + 255) / 256), block(256);
+  if (h->cfg.dof == 3)
+    hipLaunchKernelGGL(boundary_kernel<3>, grid, block, 0, h->stream, h->mesh, b);
+  else
+    hipLaunchKernelGGL(boundary_kernel<4>, grid, block, 0, h->stream, h->mesh, b);
+  HIP_TRY(hipGetLastError());
+  return MCV_OK;
+}
+
+// One RK stage: stage kernel from level `in` (base Yn = level 0) into level
+// `out`, then the boundary pass on `out`.
+int rk_stage(mcv_handle *h, int kind, int in, int out, double c1) {
+  StageArgs a{};
+  a.in_phi = h->lv[in][0]; a.in_u = h->lv[in][1]; a.in_v = h->lv[in][2];
+  a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
+  a.acc_phiG = h->acc[0]; a.acc_u = h->acc[1]; a.acc_v = h->acc[2];
+  a.o_phi = h->lv[out][0]; a.o_u = h->lv[out][1]; a.o_v = h->lv[out][2];
+  a.edge_raw = h->edge_raw;
+  if (h->first_pending) { a.cu0 = h->cu0; a.cv0 = h->cv0; }
+  a.c1 = c1;
+  int rc = stage(h, kind, a);
+  if (rc) return rc;
+  h->first_pending = false;
+  return boundary(h, out);
+}
+
+// time_scheme_mod.F90:13-69.  Levels: 0 = Yn (updated in place by the last
+// stage), 1 and 2 = intermediate stage states.
+int one_step(mcv_handle *h) {
+  const double dt = h->cfg.dt;
+  int rc;
+  if (h->cfg.scheme == MCV_RK4) {
+    if ((rc = rk_stage(h, kRK4S1, 0, 1, 0.5 * dt))) return rc;
+    if ((rc = rk_stage(h, kRK4S2, 1, 2, 0.5 * dt))) return rc;
+    if ((rc = rk_stage(h, kRK4S3, 2, 1, dt))) return rc;
+    if ((rc = rk_stage(h, kRK4S4, 1, 0, dt))) return rc;
+  } else {
+    if ((rc = rk_stage(h, kRK3S1, 0, 1, dt))) return rc;
+    if ((rc = rk_stage(h, kRK3S2, 1, 2, 0.25 * dt))) return rc;
+    if ((rc = rk_stage(h, kRK3S3, 2, 0, 2.0 / 3.0 * dt))) return rc;
+  }
+  return MCV_OK;
+}
+
+int run_diag(mcv_handle *h, double *mass, double *energy) {
+  const double *cu0 = h->first_pending ? h->cu0 : nullptr;
+  const double *cv0 = h->first_pending ? h->cv0 : nullptr;
+  dim3 grid(h->diag_blocks), block(kDiagThreads);
+  if (h->cfg.dof == 3)
+    hipLaunchKernelGGL(diag_partial_kernel<3>, grid, block, 0, h->stream, h->mesh, h->lv[0][0],
+                       h->lv[0][1], h->lv[0][2], cu0, cv0, h->partials);
+  else
+    hipLaunchKernelGGL(diag_partial_kernel<4>, grid, block, 0, h->stream, h->mesh, h->lv[0][0],
+                       h->lv[0][1], h->lv[0][2], cu0, cv0, h->partials);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(1), 0, h->stream, h->partials,
+                     h->diag_blocks, h->diag_out);
+  HIP_TRY(hipGetLastError());
+  double r[2];
+  HIP_TRY(hipMemcpyAsync(r, h->diag_out, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (mass) *mass = r[0];
+  if (energy) *energy = r[1];
+  if (!std::isfinite(r[0]) || !std::isfinite(r[1]))
+    return fail(MCV_ENONFINITE, "non-finite integrals: mass %g energy %g", r[0], r[1]);
+  return MCV_OK;
+}
+
+int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
+              mcv_handle *h) {
+  int rc = validate(cfg);
+  if (rc) return rc;
+  if (!mesh || !mesh->sqrtG || !mesh->matrixG || !mesh->matrixIG || !mesh->matrixA ||
+      !mesh->matrixIA || !mesh->f || !mesh->phi_s || !mesh->ghost_coef || !mesh->ghost_iref)
+    return fail(MCV_EINVAL, "mesh or one of its arrays is NULL");
+  if (!st || !st->phi || !st->phiG || !st->u || !st->v || !st->contraU || !st->contraV)
+    return fail(MCV_EINVAL, "initial state or one of its arrays is NULL");
+  h->cfg = *cfg;
+  const int H = cfg->dof - 1;
+  h->H = H;
+  h->npv = cfg->n * H + 1;
+  h->nh = h->npv + 2 * H;
+  const long long per = (long long)h->nh * h->nh;
+  h->np = 6 * per;
+  const long long np = h->np;
+
+  // ghost tables
+  for (long long k = 0; k < (long long)H * h->npv; ++k) {
+    const int c = mesh->ghost_iref[k];
+    if (c < 1 || c > cfg->n)
+      return fail(MCV_EINVAL, "ghost_iref[%lld] = %d outside [1, %d]", k, c, cfg->n);
+    if (!std::isfinite(mesh->ghost_coef[k]))
+      return fail(MCV_EINVAL, "ghost_coef[%lld] is not finite", k);
+  }
+  // split IG into its three distinct components; require IG21 == IG12
+  std::vector<double> ig11(np), ig12(np), ig22(np), p2sp(4 * np), sp2p(4 * np);
+  for (long long g = 0; g < np; ++g) {
+    const double *IG = mesh->matrixIG + 4 * g;
+    if (!bits_equal(IG[1], IG[2]))
+      return fail(MCV_EINVAL, "matrixIG is not symmetric at point %lld", g);
+    ig11[g] = IG[0];
+    ig12[g] = IG[2];
+    ig22[g] = IG[3];
+    // (A*IG) and (G*IA), formed first as projection_mod.F90:71,90 do
+    const double *A = mesh->matrixA + 4 * g, *G = mesh->matrixG + 4 * g,
+                 *IA = mesh->matrixIA + 4 * g;
+    double *P = p2sp.data() + 4 * g, *S = sp2p.data() + 4 * g;
+    P[0] = A[0] * IG[0] + A[2] * IG[1];
+    P[1] = A[1] * IG[0] + A[3] * IG[1];
+    P[2] = A[0] * IG[2] + A[2] * IG[3];
+    P[3] = A[1] * IG[2] + A[3] * IG[3];
+    S[0] = G[0] * IA[0] + G[2] * IA[1];
+    S[1] = G[1] * IA[0] + G[3] * IA[1];
+    S[2] = G[0] * IA[2] + G[2] * IA[3];
+    S[3] = G[1] * IA[2] + G[3] * IA[3];
+  }
+  // phiG must be phi*sqrtG (the device keeps phi only)
+  for (int p = 0; p < 6; ++p)
+    for (int j = 0; j < h->nh; ++j)
+      for (int i = 0; i < h->nh; ++i) {
+        const bool corner = (i < H || i >= H + h->npv) && (j < H || j >= H + h->npv);
+        if (corner) continue;
+        const long long g = p * per + (long long)j * h->nh + i;
+        if (!bits_equal(st->phiG[g], st->phi[g] * mesh->sqrtG[g]))
+          return fail(MCV_EINVAL, "initial phiG != phi*sqrtG at point %lld", g);
+      }
+
+  HIP_TRY(hipSetDevice(cfg->device));
+  HIP_TRY(hipStreamCreate(&h->stream));
+
+  MeshDev &m = h->mesh;
+  m.dof = cfg->dof;
+  m.n = cfg->n;
+  m.npv = h->npv;
+  m.nh = h->nh;
+  m.dx = cfg->dx;
+  const bool metric_inv = panel_invariant(mesh->sqrtG, per) && panel_invariant(ig11.data(), per) &&
+                          panel_invariant(ig12.data(), per) && panel_invariant(ig22.data(), per);
+  const long long mcount = metric_inv ? per : np;
+  m.metric_pstride = metric_inv ? 0 : per;
+  const bool phis_inv = panel_invariant(mesh->phi_s, per);
+  m.phis_pstride = phis_inv ? 0 : per;
+  double *d;
+  if ((rc = upload(h, &d, mesh->sqrtG, mcount))) return rc;
+  m.sqrtG = d;
+  if ((rc = upload(h, &d, ig11.data(), mcount))) return rc;
+  m.ig11 = d;
+  if ((rc = upload(h, &d, ig12.data(), mcount))) return rc;
+  m.ig12 = d;
+  if ((rc = upload(h, &d, ig22.data(), mcount))) return rc;
+  m.ig22 = d;
+  if ((rc = upload(h, &d, mesh->f, np))) return rc;
+  m.f = d;
+  if ((rc = upload(h, &d, mesh->phi_s, phis_inv ? per : np))) return rc;
+  m.phi_s = d;
+  if ((rc = upload(h, &d, p2sp.data(), 4 * np))) return rc;
+  m.p2sp = d;
+  if ((rc = upload(h, &d, sp2p.data(), 4 * np))) return rc;
+  m.sp2p = d;
+  if ((rc = upload(h, &d, mesh->ghost_coef, (size_t)H * h->npv))) return rc;
+  m.gcoef = d;
+  {
+    void *pi = nullptr;
+    HIP_TRY(hipMalloc(&pi, sizeof(int32_t) * H * h->npv));
+    h->allocs.push_back(pi);
+    HIP_TRY(hipMemcpy(pi, mesh->ghost_iref, sizeof(int32_t) * H * h->npv,
+                      hipMemcpyHostToDevice));
+    m.giref = static_cast<const int32_t *>(pi);
+  }
+  // state levels
+  for (int l = 0; l < 3; ++l)
+    for (int f = 0; f < 3; ++f)
+      if ((rc = dev_alloc(h, &h->lv[l][f], np))) return rc;
+  for (int f = 0; f < 3; ++f)
+    if ((rc = dev_alloc(h, &h->acc[f], np))) return rc;
+  if ((rc = dev_alloc(h, &h->edge_raw, (size_t)6 * 4 * 3 * h->npv))) return rc;
+  HIP_TRY(hipMemcpy(h->lv[0][0], st->phi, np * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->lv[0][1], st->u, np * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->lv[0][2], st->v, np * sizeof(double), hipMemcpyHostToDevice));
+  // halo corners of the scratch levels are never read, but keep them defined
+  for (int l = 1; l < 3; ++l)
+    for (int f = 0; f < 3; ++f)
+      HIP_TRY(hipMemcpy(h->lv[l][f], h->lv[0][f], np * sizeof(double), hipMemcpyDeviceToDevice));
+  if ((rc = upload(h, &h->cu0, st->contraU, np))) return rc;
+  if ((rc = upload(h, &h->cv0, st->contraV, np))) return rc;
+  h->first_pending = true;
+  // integrals
+  const long long ncell = 6LL * cfg->n * cfg->n;
+  h->diag_blocks = (int)std::min<long long>((ncell + kDiagThreads - 1) / kDiagThreads, 1024);
+  if ((rc = dev_alloc(h, &h->partials, 4 * (size_t)h->diag_blocks))) return rc;
+  if ((rc = dev_alloc(h, &h->diag_out, 2))) return rc;
+  for (int f = 0; f < 6; ++f)
+    if ((rc = dev_alloc(h, &h->pack[f], np))) return rc;
+  h->steps = 0;
+  HIP_TRY(hipDeviceSynchronize());
+  return MCV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcv_hip_init(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *initial,
+                 mcv_handle **out) {
+  g_last_error.clear();
+  if (!out) return fail(MCV_EINVAL, "out is NULL");
+  *out = nullptr;
+  mcv_handle *h = new (std::nothrow) mcv_handle();
+  if (!h) return fail(MCV_ENOMEM, "host allocation failed");
+  int rc = init_impl(cfg, mesh, initial, h);
+  if (rc) {
+    std::string msg = g_last_error;
+    destroy(h);
+    g_last_error = msg;
+    return rc;
+  }
+  *out = h;
+  return MCV_OK;
+}
+
+int mcv_hip_step(mcv_handle *h, int32_t nsteps) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (nsteps < 0) return fail(MCV_EINVAL, "nsteps must be >= 0, got %d", nsteps);
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  for (int32_t s = 0; s < nsteps; ++s) {
+    int rc = one_step(h);
+    if (rc) return rc;
+    ++h->steps;
+  }
+  return MCV_OK;
+}
+
+int mcv_hip_sync(mcv_handle *h, mcv_state *out, double *mass, double *energy) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (out) {
+    const double *cu0 = h->first_pending ? h->cu0 : nullptr;
+    const double *cv0 = h->first_pending ? h->cv0 : nullptr;
+    hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh, h->lv[0][0],
+                       h->lv[0][1], h->lv[0][2], cu0, cv0, h->pack[0], h->pack[1], h->pack[2],
+                       h->pack[3], h->pack[4], h->pack[5]);
+    HIP_TRY(hipGetLastError());
+    double *dst[6] = {out->phi, out->phiG, out->u, out->v, out->contraU, out->contraV};
+    for (int f = 0; f < 6; ++f)
+      if (dst[f])
+        HIP_TRY(hipMemcpyAsync(dst[f], h->pack[f], h->np * sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  if (mass || energy) return run_diag(h, mass, energy);
+  return MCV_OK;
+}
+
+int mcv_hip_sphere_winds(mcv_handle *h, double *zonal, double *meridional) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (!zonal || !meridional) return fail(MCV_EINVAL, "output array is NULL");
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  hipLaunchKernelGGL(sphere_winds_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh,
+                     h->lv[0][1], h->lv[0][2], h->pack[0], h->pack[1]);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(zonal, h->pack[0], h->np * sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipMemcpyAsync(meridional, h->pack[1], h->np * sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return MCV_OK;
+}
+
+int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG, double *t_u,
+                     double *t_v) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (!in || !in->phi || !in->u || !in->v || !in->contraU || !in->contraV || !t_phiG || !t_u ||
+      !t_v)
+    return fail(MCV_EINVAL, "state or output array is NULL");
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const size_t b = h->np * sizeof(double);
+  // scratch: pack buffers hold the input, the acc level receives the tendency
+  HIP_TRY(hipMemcpy(h->pack[0], in->phi, b, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->pack[2], in->u, b, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->pack[3], in->v, b, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->pack[4], in->contraU, b, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->pack[5], in->contraV, b, hipMemcpyHostToDevice));
+  StageArgs a{};
+  a.in_phi = h->pack[0]; a.in_u = h->pack[2]; a.in_v = h->pack[3];
+  a.cu0 = h->pack[4]; a.cv0 = h->pack[5];
+  double *tk[3];
+  int rc;
+  for (int f = 0; f < 3; ++f)
+    if ((rc = dev_alloc(h, &tk[f], (size_t)6 * h->npv * h->npv))) return rc;
+  a.o_phi = tk[0]; a.o_u = tk[1]; a.o_v = tk[2];
+  if ((rc = stage(h, kTend, a))) return rc;
+  const size_t tb = (size_t)6 * h->npv * h->npv * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(t_phiG, tk[0], tb, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(t_u, tk[1], tb, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(t_v, tk[2], tb, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int f = 0; f < 3; ++f) {
+    (void)hipFree(tk[f]);
+    h->allocs.pop_back();
+  }
+  return MCV_OK;
+}
+
+int mcv_hip_steps_done(const mcv_handle *h, int64_t *steps) {
+  if (!h || !steps) return fail(MCV_EINVAL, "NULL argument");
+  *steps = h->steps;
+  return MCV_OK;
+}
+
+int mcv_hip_free(mcv_handle *h) {
+  destroy(h);
+  return MCV_OK;
+}
+
+const char *mcv_hip_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+// TEST-ONLY CPU shim of the few HIP runtime names the product sources use.
+//
+// tests/emu builds mcv_sw_amd/csrc/mcv_hip.hip with g++ against this header
+// into tests/emu/_build/libmcv_hip_emu.so, which exports the same C-ABI.  The
+// CPU suite uses it to check the kernels' indexing (under AddressSanitizer)
+// and their bitwise agreement with the oracle without a GPU.  The product
+// package never loads it.
+//
+// Emulation model: a launch runs its blocks one after another, each with a
+// single thread (blockDim = 1).  Every kernel in the product loops over its
+// work with a block-strided loop, so one thread covers the whole block and
+// __syncthreads() has nothing to wait for.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#define __global__
+#define __device__
+#define __host__
+#define __constant__
+#define __shared__ static
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+
+struct dim3 {
+  unsigned x, y, z;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+inline thread_local dim3 blockIdx(0), threadIdx(0), blockDim(1), gridDim(1);
+inline void __syncthreads() {}
+using std::max;
+using std::min;
+
+typedef int hipError_t;
+typedef void *hipStream_t;
+enum { hipSuccess = 0 };
+enum hipMemcpyKind {
+  hipMemcpyHostToDevice,
+  hipMemcpyDeviceToHost,
+  hipMemcpyDeviceToDevice,
+  hipMemcpyHostToHost
+};
+inline const char *hipGetErrorString(hipError_t) { return "emulated"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline hipError_t hipStreamCreate(hipStream_t *s) { *s = (void *)1; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipMalloc(void **p, size_t n) {
+  *p = std::calloc(1, n);
+  return *p ? hipSuccess : 2;
+}
+inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) {
+  std::memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind,
+                                 hipStream_t = nullptr) {
+  std::memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t = nullptr) {
+  std::memset(d, v, n);
+  return hipSuccess;
+}
+
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)          \
+  do {                                                                       \
+    gridDim = dim3(grid);                                                    \
+    blockDim = dim3(1);                                                      \
+    threadIdx = dim3(0);                                                     \
+    for (unsigned bx_ = 0; bx_ < gridDim.x; ++bx_) {                         \
+      blockIdx = dim3(bx_);                                                  \
+      kernel(__VA_ARGS__);                                                   \
+    }                                                                        \
+  } while (0)
