@@ -36,8 +36,29 @@ def build_hip(verbose=False):
     return HIP_LIB
 
 
+FC = os.environ.get("MCV_FC", "/opt/rocm/bin/amdflang")
+HOST_DIR = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libmcv_host.so")
+# The reference's own kind semantics (REAL = REAL(8), DOUBLE PRECISION =
+# REAL(16)) so the host setup reproduces the reference geometry bit for bit.
+FFLAGS = ["-cpp", "-fdefault-real-8", "-O3", "-fPIC"]
+
+
+def build_host(verbose=False):
+    mod_dir = os.path.join(HOST_DIR, "_build")
+    os.makedirs(mod_dir, exist_ok=True)
+    tmp = HOST_LIB + ".tmp"
+    cmd = [FC] + FFLAGS + ["-shared", "-J", mod_dir, "-o", tmp,
+                           os.path.join(HOST_DIR, "mcv_host_lib.F90")]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("amdflang failed:\n" + p.stderr[-6000:])
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
+
+
 def build_all(verbose=False):
-    return [build_hip(verbose)]
+    return [build_hip(verbose), build_host(verbose)]
 
 
 if __name__ == "__main__":
