@@ -112,6 +112,20 @@ int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG,
 /* Number of completed time steps since mcv_hip_init. */
 int mcv_hip_steps_done(const mcv_handle *h, int64_t *steps);
 
+/* Measurement aid (no counterpart in the reference, whose only timer is the
+ * whole-program clock of MCV_SW.F90:33,80-82): like mcv_hip_step, but records
+ * a HIP event after every kernel on the handle's stream, blocks, and returns
+ * the summed durations of the stage kernels and of the boundary kernels in
+ * milliseconds (either pointer may be NULL). */
+int mcv_hip_step_timed(mcv_handle *h, int32_t nsteps, double *stage_ms,
+                       double *boundary_ms);
+
+/* Measurement aid (no counterpart in the reference): copies `bytes` between
+ * two device buffers `reps` times with a plain streaming kernel and returns
+ * the mean milliseconds per copy.  Used to calibrate HBM traffic counters and
+ * as the achievable-bandwidth yardstick. */
+int mcv_hip_copy_probe(int32_t device, int64_t bytes, int32_t reps, double *ms_per_copy);
+
 /* Releases the handle and its device memory.  NULL is accepted. */
 int mcv_hip_free(mcv_handle *h);
 

@@ -23,10 +23,13 @@ HIP_FLAGS = [
 ]
 
 
+HIP_SOURCES = ["mcv_hip.hip", "mcv_probe.hip"]
+
+
 def build_hip(verbose=False):
-    src = os.path.join(CSRC, "mcv_hip.hip")
+    srcs = [os.path.join(CSRC, f) for f in HIP_SOURCES]
     tmp = HIP_LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp, src]
+    cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + p.stderr[-6000:])

@@ -76,14 +76,27 @@ def load(path=None):
     L.mcv_hip_sphere_winds.argtypes = [H, _D, _D]
     L.mcv_hip_tendency.argtypes = [H, ctypes.POINTER(State), _D, _D, _D]
     L.mcv_hip_steps_done.argtypes = [H, ctypes.POINTER(ctypes.c_int64)]
+    L.mcv_hip_step_timed.argtypes = [H, ctypes.c_int32, _D, _D]
+    L.mcv_hip_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _D]
+    L.mcv_hip_copy_probe.restype = ctypes.c_int
     L.mcv_hip_free.argtypes = [H]
     L.mcv_hip_last_error.argtypes = []
     L.mcv_hip_last_error.restype = ctypes.c_char_p
     for f in ("mcv_hip_init", "mcv_hip_step", "mcv_hip_sync", "mcv_hip_sphere_winds",
-              "mcv_hip_tendency", "mcv_hip_steps_done", "mcv_hip_free"):
+              "mcv_hip_tendency", "mcv_hip_steps_done", "mcv_hip_step_timed", "mcv_hip_free"):
         getattr(L, f).restype = ctypes.c_int
     _libs[path] = L
     return L
+
+
+def copy_probe(nbytes, reps=20, device=0, lib=None):
+    """Milliseconds per plain device copy of nbytes (read nbytes + write nbytes)."""
+    L = load(lib)
+    ms = ctypes.c_double()
+    rc = L.mcv_hip_copy_probe(device, int(nbytes), int(reps), ctypes.byref(ms))
+    if rc != MCV_OK:
+        raise MCVError(rc, "mcv_hip_copy_probe failed")
+    return ms.value
 
 
 def dims(dof, n):
@@ -155,6 +168,14 @@ class Solver:
         out = [np.empty((6, self.npv, self.npv)) for _ in range(3)]
         self._check(self.lib.mcv_hip_tendency(self._h, ctypes.byref(st), *[_ptr(o) for o in out]))
         return dict(zip(("phiG", "u", "v"), out))
+
+    def step_timed(self, nsteps):
+        """Advance nsteps with a HIP event after every kernel; returns the
+        summed (stage kernel ms, boundary kernel ms)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.mcv_hip_step_timed(self._h, int(nsteps), ctypes.byref(a),
+                                                ctypes.byref(b)))
+        return a.value, b.value
 
     @property
     def steps_done(self):

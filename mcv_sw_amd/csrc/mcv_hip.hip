@@ -94,6 +94,10 @@ struct mcv_handle {
   double *pack[6] = {};
   int64_t steps = 0;
   int tiles_x = 0;
+  // per-kernel timing (mcv_hip_step_timed)
+  bool timing = false;
+  hipEvent_t ev[3] = {};
+  double stage_ms = 0.0, boundary_ms = 0.0;
 };
 
 namespace {
@@ -130,6 +134,8 @@ void destroy(mcv_handle *h) {
     (void)hipStreamSynchronize(h->stream);
     (void)hipStreamDestroy(h->stream);
   }
+  for (hipEvent_t e : h->ev)
+    if (e) (void)hipEventDestroy(e);
   for (void *p : h->allocs) (void)hipFree(p);
   delete h;
 }
@@ -204,10 +210,24 @@ int rk_stage(mcv_handle *h, int kind, int in, int out, double c1) {
   a.edge_raw = h->edge_raw;
   if (h->first_pending) { a.cu0 = h->cu0; a.cv0 = h->cv0; }
   a.c1 = c1;
+  if (h->timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
   int rc = stage(h, kind, a);
   if (rc) return rc;
   h->first_pending = false;
-  return boundary(h, out);
+  if (h->timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+  rc = boundary(h, out);
+  if (rc) return rc;
+  if (h->timing) {
+    // events bracket each kernel on the handle's stream; wait so they can be reused
+    HIP_TRY(hipEventRecord(h->ev[2], h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev[2]));
+    float a_ms = 0.f, b_ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&a_ms, h->ev[0], h->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b_ms, h->ev[1], h->ev[2]));
+    h->stage_ms += a_ms;
+    h->boundary_ms += b_ms;
+  }
+  return MCV_OK;
 }
 
 // time_scheme_mod.F90:13-69.  Levels: 0 = Yn (updated in place by the last
@@ -313,6 +333,7 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
 
   HIP_TRY(hipSetDevice(cfg->device));
   HIP_TRY(hipStreamCreate(&h->stream));
+  for (hipEvent_t &e : h->ev) HIP_TRY(hipEventCreate(&e));
 
   MeshDev &m = h->mesh;
   m.dof = cfg->dof;
@@ -486,6 +507,25 @@ int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG, double 
     (void)hipFree(tk[f]);
     h->allocs.pop_back();
   }
+  return MCV_OK;
+}
+
+int mcv_hip_step_timed(mcv_handle *h, int32_t nsteps, double *stage_ms, double *boundary_ms) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (nsteps < 0) return fail(MCV_EINVAL, "nsteps must be >= 0, got %d", nsteps);
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->timing = true;
+  h->stage_ms = h->boundary_ms = 0.0;
+  int rc = MCV_OK;
+  for (int32_t s = 0; s < nsteps && rc == MCV_OK; ++s) {
+    rc = one_step(h);
+    if (rc == MCV_OK) ++h->steps;
+  }
+  h->timing = false;
+  if (rc) return rc;
+  if (stage_ms) *stage_ms = h->stage_ms;
+  if (boundary_ms) *boundary_ms = h->boundary_ms;
   return MCV_OK;
 }
 

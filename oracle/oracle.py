@@ -39,7 +39,7 @@ def lib():
         L.orc_run.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [_D] * 8 +
                               [_I] + [_D] * 8 + [ctypes.c_int])
         L.orc_tendency.argtypes = ([ctypes.c_int, ctypes.c_int, ctypes.c_double] + [_D] * 13)
-        L.orc_boundary.argtypes = ([ctypes.c_int, ctypes.c_int, ctypes.c_double] + [_D] * 6 +
+        L.orc_boundary.argtypes = ([ctypes.c_int, ctypes.c_int, ctypes.c_double] + [_D] * 5 +
                                    [_I] + [_D] * 7)
         L.orc_diag.argtypes = [ctypes.c_int, ctypes.c_int] + [_D] * 10
         for f in ("orc_mesh_build", "orc_ghost_build", "orc_run", "orc_tendency",

@@ -16,7 +16,7 @@
 ! Inputs (current directory):
 !   namelist.input  reference namelist (parameters_mod.F90:86-99)
 !   driver.nml      &driver nsteps, ic_file, out_dir, dump_steps, history_every,
-!                   dump_mesh
+!                   dump_mesh, dump_states
 ! Outputs (out_dir): mesh.bin, ghost.bin, state_NNNNNNN.bin (raw float64,
 ! Fortran order, halo included), and one JSON timing line on stdout.
 module refdrv_scheme
@@ -141,13 +141,14 @@ program ref_driver
   implicit none
 
   integer            :: nsteps_drv, history_every, it, k, u
-  logical            :: dump_mesh, do_dump
+  logical            :: dump_mesh, dump_states, do_dump
   character(len=512) :: ic_file, out_dir
   integer            :: dump_steps(64)
   integer(8)         :: c0, c1, crate, d0, d1, dump_ticks
   real(8)            :: total_mass, total_energy
 
-  namelist /driver/ nsteps_drv, ic_file, out_dir, dump_steps, history_every, dump_mesh
+  namelist /driver/ nsteps_drv, ic_file, out_dir, dump_steps, history_every, dump_mesh, &
+                    dump_states
 
   nsteps_drv    = 1
   ic_file       = ''
@@ -155,6 +156,7 @@ program ref_driver
   dump_steps    = -1
   history_every = 0
   dump_mesh     = .true.
+  dump_states   = .true.
 
   call initParameters
   call initMesh
@@ -192,7 +194,7 @@ program ref_driver
     close(u)
   endif
 
-  call dump_state(0, stat(0))
+  if (dump_states) call dump_state(0, stat(0))
 
   dump_ticks = 0
   call system_clock(c0, crate)
@@ -211,7 +213,7 @@ program ref_driver
       endif
     endif
 
-    do_dump = (it == nsteps_drv)
+    do_dump = (it == nsteps_drv) .and. dump_states
     do k = 1, size(dump_steps)
       if (dump_steps(k) == it) do_dump = .true.
     enddo

@@ -36,8 +36,16 @@ def _fmt_deg(n):
     return s
 
 
+def _unlimited_stack():
+    # the reference keeps whole-field automatic arrays on the stack
+    import resource
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
 def run(dof, n, dt, nsteps, scheme="RK4", case_num=2, ic=None, dump_steps=(),
-        history_every=0, threads=1, workdir=None, dump_mesh=True, timeout=None):
+        history_every=0, threads=1, workdir=None, dump_mesh=True, dump_states=True,
+        timeout=None):
     """Run the reference for nsteps; returns dict(mesh, ghost, states, seconds).
 
     ic: optional dict with 'phi', 'zonal', 'merid', 'phi_s' (each (6, nh, nh))
@@ -66,10 +74,11 @@ def run(dof, n, dt, nsteps, scheme="RK4", case_num=2, ic=None, dump_steps=(),
             fh.write(f"&driver\nnsteps_drv={nsteps}\nic_file='{icf}'\nout_dir='{wd}'\n"
                      f"dump_steps={','.join(str(x) for x in ds)}\n"
                      f"history_every={history_every}\n"
-                     f"dump_mesh={'.true.' if dump_mesh else '.false.'}\n/\n")
+                     f"dump_mesh={'.true.' if dump_mesh else '.false.'}\n"
+                     f"dump_states={'.true.' if dump_states else '.false.'}\n/\n")
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         p = subprocess.run([binary(dof)], cwd=wd, env=env, capture_output=True, text=True,
-                           timeout=timeout)
+                           timeout=timeout, preexec_fn=_unlimited_stack)
         if p.returncode != 0:
             raise RuntimeError(f"ref_driver failed ({p.returncode}): {p.stderr[-2000:]}")
         timing = None

@@ -14,6 +14,15 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session")
+def emu_lib():
+    """The product sources compiled for the CPU against the test-only shim."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "emu"))
+    import build_emu
+
+    return build_emu.build_lib()
+
+
+@pytest.fixture(scope="session")
 def hip_lib():
     """The in-tree HIP library; GPU tests fail (not skip) if it is missing."""
     from mcv_sw_amd import capi

@@ -13,7 +13,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
-SRC = os.path.join(ROOT, "mcv_sw_amd", "csrc", "mcv_hip.hip")
+SRCS = [os.path.join(ROOT, "mcv_sw_amd", "csrc", f) for f in ("mcv_hip.hip", "mcv_probe.hip")]
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libmcv_hip_emu.so")
 ASAN = os.path.join(OUT, "asan_driver")
@@ -39,7 +39,7 @@ def _deps():
 def build_lib(force=False):
     os.makedirs(OUT, exist_ok=True)
     if force or _stale(LIB, _deps()):
-        subprocess.run(COMMON + ["-O2", "-fPIC", "-shared", "-x", "c++", SRC, "-o", LIB],
+        subprocess.run(COMMON + ["-O2", "-fPIC", "-shared", "-x", "c++"] + SRCS + ["-o", LIB],
                        check=True)
     return LIB
 
@@ -48,7 +48,7 @@ def build_asan(force=False):
     os.makedirs(OUT, exist_ok=True)
     if force or _stale(ASAN, _deps()):
         subprocess.run(COMMON + ["-O1", "-g", "-fsanitize=address,undefined",
-                                 "-fno-sanitize-recover=all", "-x", "c++", SRC, "-x", "c++",
+                                 "-fno-sanitize-recover=all", "-x", "c++"] + SRCS + ["-x", "c++",
                                  os.path.join(HERE, "asan_driver.cpp"), "-o", ASAN], check=True)
     return ASAN
 

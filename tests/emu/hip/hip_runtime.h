@@ -68,6 +68,22 @@ inline hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t = nullptr
   std::memset(d, v, n);
   return hipSuccess;
 }
+inline hipError_t hipMemset(void *d, int v, size_t n) {
+  std::memset(d, v, n);
+  return hipSuccess;
+}
+struct double2 {
+  double x, y;
+};
+typedef void *hipEvent_t;
+inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)1; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t = nullptr) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
+  *ms = 0.f;
+  return hipSuccess;
+}
 
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)          \
   do {                                                                       \

@@ -1,0 +1,109 @@
+"""Write the golden vectors of tests/golden from the reference build.
+
+Runs oracle/_ref/mcv{3,4}/ref_driver (the reference's own Fortran modules,
+compiled unmodified; see oracle/Makefile) in this container and records:
+  golden.json   per case: SHA-256 of the reference's arrays on every defined
+                point (interior and halo strips; the halo corners hold fill
+                or stale values in the reference and are excluded), the
+                integrals of diag_mod.F90 at the first and last step, and
+                the reference's mesh digests
+  tiny_*.npz    full step-0 and final arrays of the 4x4-cell cases
+Run: python tests/golden/make_golden.py   (needs `make -C oracle ref`)
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import ref_runner  # noqa: E402
+
+FIELDS = ("phi", "phiG", "u", "v", "contraU", "contraV")
+MESH_KEYS = ("xP", "yP", "lonP", "latP", "sqrtG", "G", "IG", "A", "IA", "f", "coef", "iref")
+
+
+def defined_mask(dof, n):
+    h = dof - 1
+    npv = n * h + 1
+    nh = npv + 2 * h
+    m = np.ones((nh, nh), bool)
+    for a in (slice(0, h), slice(h + npv, nh)):
+        for b in (slice(0, h), slice(h + npv, nh)):
+            m[a, b] = False
+    return m
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def state_digests(st, dof, n):
+    msk = defined_mask(dof, n)
+    return {k: digest(st[k][:, msk]) for k in FIELDS}
+
+
+# (name, dof, n, initial state, scheme, dt, steps, extra)
+CASES = [
+    ("tiny_mcv3_case5_rk4", 3, 4, ("case", 5), "RK4", 1200.0, 8, {}),
+    ("tiny_mcv4_case6_rk3", 4, 4, ("case", 6), "RK3_TVD", 900.0, 8, {}),
+    ("mcv3_n8_case6_rk3", 3, 8, ("case", 6), "RK3_TVD", 600.0, 20, {}),
+    ("mcv4_n12_case2_rk4", 4, 12, ("case", 2), "RK4", 300.0, 48, {}),
+    ("mcv3_n8_case5_rk4_history", 3, 8, ("case", 5), "RK4", 600.0, 10, {"history_every": 1}),
+    ("mcv3_n8_case5_rk4", 3, 8, ("case", 5), "RK4", 600.0, 10, {}),
+    ("mcv3_n32_case5_rk4_day", 3, 32, ("case", 5), "RK4", 240.0, 360, {}),
+    ("mcv3_n32_jet_rk4_day", 3, 32, ("jet", dict(u0=20.0, h0=5960.0, lmax=8, seed=0, rms_h=30.0)),
+     "RK4", 240.0, 360, {}),
+    ("mcv4_n16_jet_rk4", 4, 16, ("jet", dict(u0=40.0, h0=8000.0, lmax=16, seed=0, rms_h=60.0)),
+     "RK4", 300.0, 24, {}),
+]
+
+MESHES = [(3, 4), (4, 4), (3, 8), (3, 32), (4, 12), (4, 16), (4, 48), (3, 90)]
+
+
+def main():
+    out = {"cases": {}, "meshes": {}}
+    for dof, n in MESHES:
+        r = ref_runner.run(dof, n, dt=60.0, nsteps=0, case_num=2)
+        out["meshes"][f"{dof}_{n}"] = {k: digest(r["mesh"][k]) for k in MESH_KEYS}
+        print("mesh", dof, n, flush=True)
+    for name, dof, n, init, scheme, dt, steps, extra in CASES:
+        ic = None
+        case_num = 2
+        if init[0] == "case":
+            case_num = init[1]
+        else:
+            from mcv_sw_amd import hostlib  # the IC generator is product code
+
+            m = hostlib.mesh(dof, n)
+            st = hostlib.jet(m, **init[1])
+            ic = {"phi": st["phi"], "zonal": st["zonal"], "merid": st["merid"],
+                  "phi_s": m["phi_s"]}
+        r = ref_runner.run(dof, n, dt, steps, scheme=scheme, case_num=case_num, ic=ic,
+                           threads=8, **extra)
+        s0, s1 = r["states"][0], r["states"][steps]
+        rec = {"dof": dof, "n": n, "init": [init[0], init[1]], "scheme": scheme, "dt": dt,
+               "steps": steps, "extra": extra,
+               "step0": state_digests(s0, dof, n), "final": state_digests(s1, dof, n),
+               "mass0": s0["mass"], "energy0": s0["energy"], "mass": s1["mass"],
+               "energy": s1["energy"], "phi_s": digest(r["mesh"]["phi_s"])}
+        out["cases"][name] = rec
+        if name.startswith("tiny_"):
+            arrs = {}
+            for k in FIELDS:
+                arrs["s0_" + k] = s0[k]
+                arrs["s1_" + k] = s1[k]
+            arrs["s0_zonal"], arrs["s0_merid"] = s0["zonal"], s0["merid"]
+            arrs["phi_s"] = r["mesh"]["phi_s"]
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrs)
+        print("case", name, flush=True)
+    with open(os.path.join(HERE, "golden.json"), "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+"""Pins against the reference build's golden vectors (tests/golden).
+
+golden.json holds SHA-256 digests of the reference's own arrays (mesh, step-0
+state, final state on every defined point) written by make_golden.py from
+oracle/_ref.  Three implementations are checked against them bit for bit:
+  - the oracle's C restatement (mesh, ghost tables, trajectories),
+  - the product's Fortran host library (mesh, ghost tables, initial states),
+  - the product's HIP sources compiled for the CPU by tests/emu (trajectories).
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from mcv_sw_amd import hostlib
+from oracle import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+FIELDS = ("phi", "phiG", "u", "v", "contraU", "contraV")
+MESH_KEYS = ("xP", "yP", "lonP", "latP", "sqrtG", "G", "IG", "A", "IA", "f", "coef", "iref")
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def state_digests(st, dof, n):
+    msk = cases.masks(dof, n)["defined"]
+    return {k: digest(st[k][:, msk]) for k in FIELDS}
+
+
+@pytest.mark.parametrize("key", sorted(GOLD["meshes"]))
+def test_oracle_mesh_matches_reference(key):
+    dof, n = map(int, key.split("_"))
+    m = oracle.mesh(dof, n)
+    got = {k: digest(m[k]) for k in MESH_KEYS}
+    bad = [k for k in MESH_KEYS if got[k] != GOLD["meshes"][key][k]]
+    assert not bad, f"oracle mesh differs from the reference in {bad}"
+
+
+@pytest.mark.parametrize("key", sorted(GOLD["meshes"]))
+def test_hostlib_mesh_matches_reference(key):
+    dof, n = map(int, key.split("_"))
+    m = hostlib.mesh(dof, n)
+    got = {k: digest(m[k]) for k in MESH_KEYS}
+    bad = [k for k in MESH_KEYS if got[k] != GOLD["meshes"][key][k]]
+    assert not bad, f"host-library mesh differs from the reference in {bad}"
+
+
+def _initial(rec):
+    """Initial state of a golden case, built by the product's host library."""
+    dof, n = rec["dof"], rec["n"]
+    m = hostlib.mesh(dof, n)
+    kind, arg = rec["init"]
+    st = hostlib.williamson(m, arg) if kind == "case" else hostlib.jet(m, **arg)
+    return m, st
+
+
+BITWISE_IC = [k for k, r in GOLD["cases"].items()
+              if not (r["init"][0] == "case" and r["init"][1] == 6)]
+
+
+@pytest.mark.parametrize("name", sorted(BITWISE_IC))
+def test_hostlib_initial_state_matches_reference(name):
+    rec = GOLD["cases"][name]
+    m, st = _initial(rec)
+    assert state_digests(st, rec["dof"], rec["n"]) == rec["step0"]
+    assert digest(m["phi_s"]) == rec["phi_s"]
+
+
+def test_hostlib_case6_within_one_ulp():
+    """Case 6: flang pairs a sin/cos differently in one statement; 1 ulp at most."""
+    z = np.load(os.path.join(HERE, "golden", "tiny_mcv4_case6_rk3.npz"))
+    m = hostlib.mesh(4, 4)
+    st = hostlib.williamson(m, 6)
+    msk = cases.masks(4, 4)["defined"]
+    for k in FIELDS:
+        a, b = st[k][:, msk], z["s0_" + k][:, msk]
+        ulp = np.abs(a - b) / np.spacing(np.abs(b))
+        assert ulp.max() <= 1.0, k
+
+
+def _ref_initial(name, rec):
+    """Step-0 state exactly as the reference had it."""
+    if name.startswith("tiny_"):
+        z = np.load(os.path.join(HERE, "golden", name + ".npz"))
+        m = hostlib.mesh(rec["dof"], rec["n"])
+        m["phi_s"] = z["phi_s"]
+        st = {k: z["s0_" + k] for k in FIELDS}
+        st["zonal"], st["merid"] = z["s0_zonal"], z["s0_merid"]
+        return m, st
+    assert name in BITWISE_IC
+    return _initial(rec)
+
+
+TRAJ = [k for k in GOLD["cases"] if k.startswith("tiny_") or k in BITWISE_IC]
+
+
+@pytest.mark.parametrize("name", sorted(TRAJ))
+def test_oracle_trajectory_matches_reference(name):
+    rec = GOLD["cases"][name]
+    m, st = _ref_initial(name, rec)
+    out = oracle.run(rec["dof"], rec["n"], 4 if rec["scheme"] == "RK4" else 3, rec["dt"], m, st,
+                     rec["steps"])
+    assert state_digests(out, rec["dof"], rec["n"]) == rec["final"]
+    mass, energy = oracle.diag(rec["dof"], rec["n"], m, out)
+    assert abs(mass - rec["mass"]) <= 1e-14 * abs(rec["mass"])
+    assert abs(energy - rec["energy"]) <= 1e-14 * abs(rec["energy"])
+
+
+def test_history_output_leaves_trajectory_unchanged():
+    """MCV_SW.F90:62-74 rewrites halo corners and sphere winds at output
+    times; the reference trajectory is the same with output every step."""
+    a = GOLD["cases"]["mcv3_n8_case5_rk4_history"]["final"]
+    b = GOLD["cases"]["mcv3_n8_case5_rk4"]["final"]
+    assert a == b
+
+
+@pytest.mark.parametrize("name", ["tiny_mcv3_case5_rk4", "tiny_mcv4_case6_rk3", "mcv3_n8_case5_rk4",
+                                  "mcv4_n12_case2_rk4", "mcv4_n16_jet_rk4"])
+def test_emulated_product_matches_reference(name, emu_lib):
+    """The product's HIP sources, compiled for the CPU, reproduce the reference."""
+    from mcv_sw_amd import capi
+
+    rec = GOLD["cases"][name]
+    m, st = _ref_initial(name, rec)
+    with capi.Solver(rec["dof"], rec["n"], rec["dt"], m, st, scheme=rec["scheme"],
+                     lib=emu_lib) as s:
+        s.step(rec["steps"])
+        out = s.sync()
+    assert state_digests(out, rec["dof"], rec["n"]) == rec["final"]
+    assert abs(out["mass"] - rec["mass"]) <= 1e-14 * abs(rec["mass"])
