@@ -44,6 +44,19 @@ def words_per_point(scheme):
     return 51 if scheme == "RK4" else 24
 
 
+def measured_traffic(path):
+    """Calibrated HBM bytes per stage launch from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/), or None."""
+    import re
+
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"Per RK4 step: (\d+(?:\.\d+)?) MB measured", text)
+    return float(m.group(1)) * 1e6 / 4 if m else None
+
+
 def cpu_baseline(mesh_fn, steps=2):
     """Reference Fortran (oracle/_ref) on this host, `steps` steps of the workload;
     the oracle's C restatement (one core) if the reference build cannot run."""
@@ -147,7 +160,10 @@ def main():
                    "dof": DOF, "n": N, "cells": cells, "scheme": SCHEME, "dt_s": DT,
                    "h_rel_err_vs_fp64": 0.0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": measured_traffic(os.path.join(
+                         ROOT, "profiles", "stage_kernel_mcv4_n384_fp64.md")),
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
                      "kernel": "stage_kernel (4 launches per RK4 step)",
                      "bytes_per_cell_step": bpcs,
                      "stage_kernel_ms_per_step": stage_ms / nt,
