@@ -126,6 +126,9 @@ def main():
         del st
         log("solver initialised")
         s.step(args.warmup)
+        tile, tile_ms = s.autotune(reps=5)
+        log(f"autotuned tile shape {tile}: " + ", ".join(f"{x:.3f}" for x in tile_ms) + " ms")
+        s.step(2)
         s.sync(fields=(), integrals=False)
         t = time.perf_counter()
         s.step(args.steps)
@@ -167,6 +170,8 @@ def main():
                      "kernel": "stage_kernel (4 launches per RK4 step)",
                      "bytes_per_cell_step": bpcs,
                      "stage_kernel_ms_per_step": stage_ms / nt,
+                     "tile_variant": tile,
+                     "tile_variant_ms_per_stage": tile_ms,
                      "boundary_kernel_ms_per_step": bnd_ms / nt},
         "mass": out["mass"],
     }

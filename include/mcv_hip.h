@@ -120,6 +120,18 @@ int mcv_hip_steps_done(const mcv_handle *h, int64_t *steps);
 int mcv_hip_step_timed(mcv_handle *h, int32_t nsteps, double *stage_ms,
                        double *boundary_ms);
 
+/* Tile-shape autotuner (no counterpart in the reference): times each of the
+ * library's MCV_TILE_VARIANTS stage-kernel tile shapes for this solve (reps
+ * launches each, on scratch levels, so the trajectory is not changed), keeps
+ * the fastest for later steps and returns its index and, if ms_per_stage is
+ * not NULL, the mean milliseconds of every variant (MCV_TILE_VARIANTS
+ * doubles).  Blocks. */
+#define MCV_TILE_VARIANTS 4
+int mcv_hip_autotune(mcv_handle *h, int32_t reps, int32_t *tile, double *ms_per_stage);
+
+/* Selects a stage-kernel tile shape by index (0 is the default). */
+int mcv_hip_set_tile(mcv_handle *h, int32_t tile);
+
 /* Measurement aid (no counterpart in the reference): copies `bytes` between
  * two device buffers `reps` times with a plain streaming kernel and returns
  * the mean milliseconds per copy.  Used to calibrate HBM traffic counters and

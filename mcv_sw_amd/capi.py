@@ -27,6 +27,7 @@ MCV_OK, MCV_EINVAL, MCV_ENOMEM, MCV_EDEVICE, MCV_ENONFINITE, MCV_ESTATE = 0, -1,
 RK3_TVD, RK4 = 3, 4
 FP64, FP32C = 0, 1
 SCHEMES = {"RK3_TVD": RK3_TVD, "RK4": RK4}
+TILE_VARIANTS = 4
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -79,6 +80,10 @@ def load(path=None):
     L.mcv_hip_step_timed.argtypes = [H, ctypes.c_int32, _D, _D]
     L.mcv_hip_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _D]
     L.mcv_hip_copy_probe.restype = ctypes.c_int
+    L.mcv_hip_autotune.argtypes = [H, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _D]
+    L.mcv_hip_autotune.restype = ctypes.c_int
+    L.mcv_hip_set_tile.argtypes = [H, ctypes.c_int32]
+    L.mcv_hip_set_tile.restype = ctypes.c_int
     L.mcv_hip_free.argtypes = [H]
     L.mcv_hip_last_error.argtypes = []
     L.mcv_hip_last_error.restype = ctypes.c_char_p
@@ -168,6 +173,17 @@ class Solver:
         out = [np.empty((6, self.npv, self.npv)) for _ in range(3)]
         self._check(self.lib.mcv_hip_tendency(self._h, ctypes.byref(st), *[_ptr(o) for o in out]))
         return dict(zip(("phiG", "u", "v"), out))
+
+    def autotune(self, reps=5):
+        """Time the stage-kernel tile shapes on this solve, keep the fastest;
+        returns (chosen index, [ms per stage launch of each shape])."""
+        t = ctypes.c_int32()
+        ms = (ctypes.c_double * TILE_VARIANTS)()
+        self._check(self.lib.mcv_hip_autotune(self._h, int(reps), ctypes.byref(t), ms))
+        return t.value, list(ms)
+
+    def set_tile(self, tile):
+        self._check(self.lib.mcv_hip_set_tile(self._h, int(tile)))
 
     def step_timed(self, nsteps):
         """Advance nsteps with a HIP event after every kernel; returns the

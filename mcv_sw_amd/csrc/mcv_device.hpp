@@ -13,6 +13,33 @@
 namespace mcv {
 
 // ---------------------------------------------------------------------------
+// Correctly rounded division by a divisor b whose reciprocal y = RN(1/b) is
+// known: q = RN(a*y), r = a - b*q (exact through the FMA), RN(q + r*y) is
+// RN(a/b) (Markstein, IBM J. Res. Dev. 34(1), 1990; y within half an ulp of
+// 1/b, q within an ulp of a/b).  Three instructions instead of the ~10 of a
+// general IEEE division.  r == 0 means q is exact (this also keeps the sign
+// of a zero quotient).  Valid away from overflow and underflow.
+// ---------------------------------------------------------------------------
+struct Recip {
+  double b, y;  // divisor and RN(1/b)
+};
+
+__host__ __device__ inline Recip make_recip(double b) { return Recip{b, 1.0 / b}; }
+
+__device__ __forceinline__ double div_by(double a, Recip d) {
+  const double q = a * d.y;
+  const double r = fma(-q, d.b, a);
+  return r == 0.0 ? q : fma(r, d.y, q);
+}
+
+// Divisors of the MCV formulas, one set per resolution.
+struct MCVDivisors {
+  Recip dx;   // dh of polyfit (MCV3), fxVIA
+  Recip dx2;  // 2*dh of polyfit (MCV4)
+  Recip dxx;  // dx**2 of fxxc (MCV4)
+};
+
+// ---------------------------------------------------------------------------
 // Per-resolution geometry and per-instance state, as the kernels see them.
 // ---------------------------------------------------------------------------
 
@@ -24,9 +51,10 @@ namespace mcv {
 struct MeshDev {
   int dof, n, npv, nh;
   double dx;                 // element width, radians
-  long long metric_pstride;  // panel stride of sqrtG, ig11, ig12, ig22
+  MCVDivisors dv;            // dx, 2dx, dx^2 with their reciprocals
+  long long metric_pstride;  // panel stride of sqrtG, rsqrtG, ig11, ig12, ig22
   long long phis_pstride;    // panel stride of phi_s
-  const double *sqrtG, *ig11, *ig12, *ig22;
+  const double *sqrtG, *rsqrtG, *ig11, *ig12, *ig22;  // rsqrtG = RN(1/sqrtG)
   const double *f;           // per panel (f differs between panels 5 and 6)
   const double *phi_s;
   const double *p2sp;        // (A*IG) per point, 4 doubles (m11, m21, m12, m22)
@@ -99,21 +127,21 @@ extern __constant__ VtxPt kVerts[8][3];
 
 // polyfitL: one-sided derivative at the cell's left end (spatial_operators_mod.F90:204-218)
 template <int DOF>
-__device__ __forceinline__ double fit_left(const double *pv, double dh) {
+__device__ __forceinline__ double fit_left(const double *pv, const MCVDivisors &dv) {
   if constexpr (DOF == 3) {
-    return -((pv[2] - 4.0 * pv[1] + 3.0 * pv[0]) / dh);
+    return -div_by(pv[2] - 4.0 * pv[1] + 3.0 * pv[0], dv.dx);
   } else {
-    return (-(11.0 * pv[0]) + 18.0 * pv[1] - 9.0 * pv[2] + 2.0 * pv[3]) / (2.0 * dh);
+    return div_by(-(11.0 * pv[0]) + 18.0 * pv[1] - 9.0 * pv[2] + 2.0 * pv[3], dv.dx2);
   }
 }
 
 // polyfitR: one-sided derivative at the cell's right end (spatial_operators_mod.F90:220-234)
 template <int DOF>
-__device__ __forceinline__ double fit_right(const double *pv, double dh) {
+__device__ __forceinline__ double fit_right(const double *pv, const MCVDivisors &dv) {
   if constexpr (DOF == 3) {
-    return (pv[0] - 4.0 * pv[1] + 3.0 * pv[2]) / dh;
+    return div_by(pv[0] - 4.0 * pv[1] + 3.0 * pv[2], dv.dx);
   } else {
-    return (11.0 * pv[3] - 18.0 * pv[2] + 9.0 * pv[1] - 2.0 * pv[0]) / (2.0 * dh);
+    return div_by(11.0 * pv[3] - 18.0 * pv[2] + 9.0 * pv[1] - 2.0 * pv[0], dv.dx2);
   }
 }
 
@@ -130,25 +158,33 @@ __device__ __forceinline__ double wave_speed(double c, double phi, double gii) {
   return a1 > a2 ? a1 : a2;
 }
 
-// sign of the contravariant wind with 0 -> +1 (spatial_operators_mod.F90:83-85)
+// c/|c| with 0 -> +1 (spatial_operators_mod.F90:83-85): exactly +-1 for every
+// finite c, NaN for NaN.
 __device__ __forceinline__ double upwind_sign(double c) {
-  if (c == 0.0) c = 1.0;
-  return c / fabs(c);
+  if (c != c) return c;
+  return c == 0.0 ? 1.0 : copysign(1.0, c);
 }
 
 // Interior-point derivative of a cell from its point values and the two
 // cell-end derivatives (spatial_operators_mod.F90:178-200).  pos in [1, H-1].
 template <int DOF>
 __device__ __forceinline__ double cell_interior(const double *f, double dL, double dR,
-                                                int pos, double dx) {
+                                                int pos, double dx, const MCVDivisors &dv) {
   if constexpr (DOF == 3) {
-    double via = (f[2] - f[0]) / dx;
+    double via = div_by(f[2] - f[0], dv.dx);
     return 1.5 * via - 0.25 * (dL + dR);
   } else {
-    double via = (f[3] - f[0]) / dx;
-    double fxx = 4.5 * (f[0] - f[1] - f[2] + f[3]) / (dx * dx);
-    if (pos == 1) return 4.0 / 3.0 * via - (4.0 * dL + 5.0 * dR) / 27.0 - 4.0 / 27.0 * dx * fxx;
-    return 4.0 / 3.0 * via - (5.0 * dL + 4.0 * dR) / 27.0 + 4.0 / 27.0 * dx * fxx;
+    // pos 1: 4/3 via - (4 dL + 5 dR)/27 - 4/27 dx fxx
+    // pos 2: 4/3 via - (5 dL + 4 dR)/27 + 4/27 dx fxx
+    // written with selects so lanes at either position take the same path
+    constexpr Recip k27{27.0, 1.0 / 27.0};
+    const double via = div_by(f[3] - f[0], dv.dx);
+    const double fxx = div_by(4.5 * (f[0] - f[1] - f[2] + f[3]), dv.dxx);
+    const bool first = pos == 1;
+    const double cl = first ? 4.0 : 5.0, cr = first ? 5.0 : 4.0;
+    const double a = 4.0 / 3.0 * via - div_by(cl * dL + cr * dR, k27);
+    const double c = 4.0 / 27.0 * dx * fxx;
+    return first ? a - c : a + c;
   }
 }
 

@@ -1,0 +1,547 @@
+// FP32-compensated (FP32C) form of the MCV step.
+//
+// Storage: the geopotential is kept as an FP32 deviation phi' = phi - phi_ref
+// from an FP64 reference geopotential (the solve's initial phi after one
+// boundary pass, so that it is continuous across panel edges and its halo is
+// the interpolation of its neighbours); u and v are kept in FP32.  The RK
+// base state Y_n carries an FP32 compensation word per field (Y = hi + lo),
+// so the long-time accumulation is done in ~48-bit precision.
+//
+// Arithmetic: the stencil (one-sided fits, Riemann fluxes, interior
+// derivatives) runs in FP32 on E' = phi' + K, on the mass flux and on the
+// winds.  The large balanced part of the pressure gradient, the MCV
+// derivative of E_ref = phi_ref + phi_s, is computed once per solve in FP64.
+// The tendencies are assembled and the RK update is done in FP64 (on gfx950
+// plain FP32 and FP64 vector instructions issue at the same rate), then
+// rounded back to FP32 hi/lo.  Metric terms, ghost weights and the
+// mass/energy integrals stay FP64.
+#pragma once
+
+#include "mcv_device.hpp"
+
+namespace mcv {
+
+// Static data of an FP32C solve (per resolution and reference state).
+struct MeshF32 {
+  int dof, n, npv, nh;
+  float rdx, r2dx, rdxx;        // 1/dx, 1/(2 dx), 1/dx^2
+  float dxf;                    // dx
+  long long metric_pstride;
+  const double *sg, *rsg, *ig11, *ig12, *ig22;  // FP64 metric (same arrays as the FP64 path)
+  const double *f;              // Coriolis parameter (FP64)
+  const float *pgref;           // RN32(phi_ref * sqrtG)
+  const double *dEx, *dEy;      // MCV derivatives of phi_ref + phi_s (FP64, owned points)
+};
+
+struct StageArgs32 {
+  const float *in_phi, *in_u, *in_v;       // stage state (phi', u, v)
+  const float *b_phi, *b_u, *b_v;          // base Y_n (hi)
+  const float *c_phi, *c_u, *c_v;          // base Y_n compensation (lo)
+  float *acc_phiG, *acc_u, *acc_v;         // RK4 accumulator
+  float *o_phi, *o_u, *o_v;                // stage output (raw interior)
+  float *oc_phi, *oc_u, *oc_v;             // compensation of the output (last stage only)
+  float *edge_raw;                         // raw edge lines (hi)
+  float *edge_raw_c;                       // raw edge lines (lo), last stage only
+  double c1;                               // stage increment coefficient (x dt)
+};
+
+__device__ __forceinline__ float fit_left_f(const float *pv, int dof, float r, float r2) {
+  if (dof == 3) return -((pv[2] - 4.0f * pv[1] + 3.0f * pv[0]) * r);
+  return (-11.0f * pv[0] + 18.0f * pv[1] - 9.0f * pv[2] + 2.0f * pv[3]) * r2;
+}
+__device__ __forceinline__ float fit_right_f(const float *pv, int dof, float r, float r2) {
+  if (dof == 3) return (pv[0] - 4.0f * pv[1] + 3.0f * pv[2]) * r;
+  return (11.0f * pv[3] - 18.0f * pv[2] + 9.0f * pv[1] - 2.0f * pv[0]) * r2;
+}
+__device__ __forceinline__ float riemann_f(float fxL, float fxR, float qxL, float qxR, float lam) {
+  return 0.5f * ((fxL + fxR) - lam * (qxR - qxL));
+}
+template <int DOF>
+__device__ __forceinline__ float cell_interior_f(const float *f, float dL, float dR, int pos,
+                                                 float dxf, float rdx, float rdxx) {
+  if constexpr (DOF == 3) {
+    return 1.5f * ((f[2] - f[0]) * rdx) - 0.25f * (dL + dR);
+  } else {
+    const float via = (f[3] - f[0]) * rdx;
+    const float fxx = 4.5f * (f[0] - f[1] - f[2] + f[3]) * rdxx;
+    const bool first = pos == 1;
+    const float cl = first ? 4.0f : 5.0f, cr = first ? 5.0f : 4.0f;
+    const float a = (4.0f / 3.0f) * via - (cl * dL + cr * dR) * (1.0f / 27.0f);
+    const float c = (4.0f / 27.0f) * dxf * fxx;
+    return first ? a - c : a + c;
+  }
+}
+
+template <int DOF, int KIND, int CX, int CY>
+__global__ __launch_bounds__(256) void stage_kernel_f32(MeshF32 m, StageArgs32 a, int tiles_x,
+                                                        int ncells) {
+  constexpr int H = DOF - 1;
+  constexpr int NX = CX * H + 1, NY = CY * H + 1;
+  constexpr int TX = NX + 2 * H, TY = NY + 2 * H;
+  __shared__ float sE[TX * TY], sU[TX * TY], sV[TX * TY], sPG[TX * TY], sCU[TX * TY],
+      sCV[TX * TY];
+  __shared__ float lamx[(CX + 1) * NY], lamy[(CY + 1) * NX];
+  __shared__ float rxF[(CX + 1) * NY], rxD[(CX + 1) * NY], rxV[(CX + 1) * NY];
+  __shared__ float ryF[(CY + 1) * NX], ryD[(CY + 1) * NX], ryU[(CY + 1) * NX];
+
+  const int npv = m.npv, nh = m.nh, N = ncells;
+  const float rdx = m.rdx, r2dx = m.r2dx, rdxx = m.rdxx, dxf = m.dxf;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int p = L % 6 + 1;
+  const int tile = L / 6;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  const int cx0 = tx * CX, cy0 = ty * CY;
+  const int ncx = min(CX, N - cx0), ncy = min(CY, N - cy0);
+  const int i0 = cx0 * H + 1, j0 = cy0 * H + 1;
+  const int nx = ncx * H + 1, ny = ncy * H + 1;
+  const int TXo = ncx * H + (cx0 + ncx == N ? 1 : 0);
+  const int TYo = ncy * H + (cy0 + ncy == N ? 1 : 0);
+  const int lw = nx + 2 * H, lh = ny + 2 * H;
+  const long long pbase = (long long)(p - 1) * nh * nh;
+  const long long mbase = (long long)(p - 1) * m.metric_pstride;
+
+  for (int t = threadIdx.x; t < TX * TY; t += blockDim.x) {
+    const int ly = t / TX, lx = t - ly * TX;
+    if (lx >= lw || ly >= lh) continue;
+    const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
+    if (!inx && !iny) continue;
+    const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
+    const long long g = pbase + off;
+    const float dp = a.in_phi[g], u = a.in_u[g], v = a.in_v[g];
+    // metric products in FP64 (the FP32 inputs are exact in FP64), rounded once
+    const double ud = u, vd = v;
+    const double g11 = m.ig11[mbase + off], g12 = m.ig12[mbase + off],
+                 g22 = m.ig22[mbase + off];
+    const double cud = g11 * ud + g12 * vd, cvd = g12 * ud + g22 * vd;
+    const double Kd = 0.5 * (cud * ud + cvd * vd);
+    const float pg = (float)((double)m.pgref[g] + (double)dp * m.sg[mbase + off]);
+    const float cu = (float)cud, cv = (float)cvd;
+    const int li = ly * TX + lx;
+    sE[li] = (float)((double)dp + Kd);  // E - E_ref
+    sU[li] = u;
+    sV[li] = v;
+    sPG[li] = pg;
+    sCU[li] = cu;
+    sCV[li] = cv;
+    if (inx && iny) {
+      const int ox = lx - H, oy = ly - H;
+      const float phi = (float)((double)pg * m.rsg[mbase + off]);
+      if (ox % H == 0) {
+        const float s = sqrtf((float)g11 * phi);
+        lamx[(ox / H) * NY + oy] = fmaxf(fabsf(cu - s), fabsf(cu + s));
+      }
+      if (oy % H == 0) {
+        const float s = sqrtf((float)g22 * phi);
+        lamy[(oy / H) * NX + ox] = fmaxf(fabsf(cv - s), fabsf(cv + s));
+      }
+    }
+  }
+  __syncthreads();
+
+  constexpr int kTasksX = (CX + 1) * NY, kTasksY = (CY + 1) * NX;
+  for (int t = threadIdx.x; t < kTasksX + kTasksY; t += blockDim.x) {
+    const bool isx = t < kTasksX;
+    const int tt = isx ? t : t - kTasksX;
+    int mb, o;
+    if (isx) { mb = tt / NY; o = tt - mb * NY; }
+    else { mb = tt / NX; o = tt - mb * NX; }
+    if (isx ? (mb > ncx || o >= ny) : (mb > ncy || o >= nx)) continue;
+    const int st = isx ? 1 : TX;
+    const int b = isx ? (o + H) * TX + (H + mb * H) : (H + mb * H) * TX + (o + H);
+    const float *sC = isx ? sCU : sCV;
+    const float *sQ = isx ? sU : sV;
+    const float *sW = isx ? sV : sU;
+    float eL[DOF], eR[DOF], qL[DOF], qR[DOF], dL[DOF], dR[DOF], pL[DOF], pR[DOF], wL[DOF],
+        wR[DOF];
+#pragma unroll
+    for (int k = 0; k < DOF; ++k) {
+      const int l = b - H * st + k * st, r = b + k * st;
+      eL[k] = sE[l];  eR[k] = sE[r];
+      qL[k] = sQ[l];  qR[k] = sQ[r];
+      pL[k] = sPG[l]; pR[k] = sPG[r];
+      dL[k] = sPG[l] * sC[l];
+      dR[k] = sPG[r] * sC[r];
+      wL[k] = sW[l];  wR[k] = sW[r];
+    }
+    const float lam = isx ? lamx[tt] : lamy[tt];
+    const float c = sC[b];
+    const float up = c == 0.0f ? 1.0f : copysignf(1.0f, c);
+    const float fF = riemann_f(fit_right_f(eL, DOF, rdx, r2dx), fit_left_f(eR, DOF, rdx, r2dx),
+                               fit_right_f(qL, DOF, rdx, r2dx), fit_left_f(qR, DOF, rdx, r2dx),
+                               lam);
+    const float fD = riemann_f(fit_right_f(dL, DOF, rdx, r2dx), fit_left_f(dR, DOF, rdx, r2dx),
+                               fit_right_f(pL, DOF, rdx, r2dx), fit_left_f(pR, DOF, rdx, r2dx),
+                               lam);
+    const float wlr = fit_right_f(wL, DOF, rdx, r2dx), wrl = fit_left_f(wR, DOF, rdx, r2dx);
+    const float fW = riemann_f(wlr, wrl, wlr, wrl, up);
+    if (isx) {
+      rxF[tt] = fF; rxD[tt] = fD; rxV[tt] = fW;
+    } else {
+      ryF[tt] = fF; ryD[tt] = fD; ryU[tt] = fW;
+    }
+  }
+  __syncthreads();
+
+  constexpr int kCells = CX * CY, kSlots = H * H * kCells, kTail = NY + NX - 1;
+  for (int t = threadIdx.x; t < kSlots + kTail; t += blockDim.x) {
+    int ox, oy;
+    if (t < kSlots) {
+      const int cls = t / kCells, cell = t - cls * kCells;
+      const int cyl = cell / CX, cxl = cell - cyl * CX;
+      if (cxl >= ncx || cyl >= ncy) continue;
+      ox = cxl * H + cls % H;
+      oy = cyl * H + cls / H;
+    } else if (t < kSlots + NY) {
+      oy = t - kSlots;
+      ox = ncx * H;
+      if (ox >= TXo || oy >= TYo) continue;
+    } else {
+      ox = t - kSlots - NY;
+      oy = ncy * H;
+      if (ox >= ncx * H || oy >= TYo) continue;
+    }
+    const int lx = ox + H, ly = oy + H, li = ly * TX + lx;
+    float fx, dvx, vx, fy, dvy, uy;
+    {
+      const int mb = ox / H, pos = ox - mb * H;
+      const int r0 = mb * NY + oy;
+      if (pos == 0) {
+        fx = rxF[r0]; dvx = rxD[r0]; vx = rxV[r0];
+      } else {
+        const int b0 = ly * TX + H + mb * H;
+        float fe[DOF], fd[DOF], fv[DOF];
+#pragma unroll
+        for (int k = 0; k < DOF; ++k) {
+          fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
+        }
+        const int r1 = r0 + NY;
+        fx = cell_interior_f<DOF>(fe, rxF[r0], rxF[r1], pos, dxf, rdx, rdxx);
+        dvx = cell_interior_f<DOF>(fd, rxD[r0], rxD[r1], pos, dxf, rdx, rdxx);
+        vx = cell_interior_f<DOF>(fv, rxV[r0], rxV[r1], pos, dxf, rdx, rdxx);
+      }
+    }
+    {
+      const int mb = oy / H, pos = oy - mb * H;
+      const int r0 = mb * NX + ox;
+      if (pos == 0) {
+        fy = ryF[r0]; dvy = ryD[r0]; uy = ryU[r0];
+      } else {
+        const int b0 = (H + mb * H) * TX + lx;
+        float fe[DOF], fd[DOF], fu[DOF];
+#pragma unroll
+        for (int k = 0; k < DOF; ++k) {
+          const int l = b0 + k * TX;
+          fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
+        }
+        const int r1 = r0 + NX;
+        fy = cell_interior_f<DOF>(fe, ryF[r0], ryF[r1], pos, dxf, rdx, rdxx);
+        dvy = cell_interior_f<DOF>(fd, ryD[r0], ryD[r1], pos, dxf, rdx, rdxx);
+        uy = cell_interior_f<DOF>(fu, ryU[r0], ryU[r1], pos, dxf, rdx, rdxx);
+      }
+    }
+    const int i = i0 + ox, j = j0 + oy;
+    const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
+    const long long g = pbase + off;
+    // tendencies assembled in FP64 (spatial_operators_mod.F90:127-131)
+    const double sg = m.sg[mbase + off], rsg = m.rsg[mbase + off];
+    const double fc = m.f[g];
+    const double cu = sCU[li], cv = sCV[li];
+    const double vort = ((double)vx - (double)uy) * rsg;
+    const double kpg = -(double)dvx - (double)dvy;
+    const double ku = -(m.dEx[g] + (double)fx) + sg * (vort + fc) * cv;
+    const double kv = -(m.dEy[g] + (double)fy) - sg * (vort + fc) * cu;
+    const double kph = kpg * rsg;  // d(phi')/dt = d(phiG)/dt / sqrtG
+    if constexpr (KIND == kTend) {
+      const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
+      a.o_phi[ti] = (float)kpg;
+      a.o_u[ti] = (float)ku;
+      a.o_v[ti] = (float)kv;
+    } else {
+      // base state Y_n = hi + lo in FP64
+      double bph, bu, bv;
+      if constexpr (KIND == kRK4S1 || KIND == kRK3S1) {
+        bph = (double)a.in_phi[g] + (double)a.c_phi[g];
+        bu = (double)sU[li] + (double)a.c_u[g];
+        bv = (double)sV[li] + (double)a.c_v[g];
+      } else {
+        bph = (double)a.b_phi[g] + (double)a.c_phi[g];
+        bu = (double)a.b_u[g] + (double)a.c_u[g];
+        bv = (double)a.b_v[g] + (double)a.c_v[g];
+      }
+      double ph, u, v;
+      if constexpr (KIND == kRK4S1) {
+        ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
+        a.acc_phiG[g] = (float)kph; a.acc_u[g] = (float)ku; a.acc_v[g] = (float)kv;
+      } else if constexpr (KIND == kRK4S2 || KIND == kRK4S3) {
+        ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
+        a.acc_phiG[g] = (float)((double)a.acc_phiG[g] + 2.0 * kph);
+        a.acc_u[g] = (float)((double)a.acc_u[g] + 2.0 * ku);
+        a.acc_v[g] = (float)((double)a.acc_v[g] + 2.0 * kv);
+      } else if constexpr (KIND == kRK4S4) {
+        ph = bph + a.c1 * (((double)a.acc_phiG[g] + kph) * (1.0 / 6.0));
+        u = bu + a.c1 * (((double)a.acc_u[g] + ku) * (1.0 / 6.0));
+        v = bv + a.c1 * (((double)a.acc_v[g] + kv) * (1.0 / 6.0));
+      } else if constexpr (KIND == kRK3S1) {
+        ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
+      } else if constexpr (KIND == kRK3S2) {  // Yn + 1/4 (Y1 - Yn) + 1/4 dt k
+        ph = bph + 0.25 * ((double)a.in_phi[g] - bph) + a.c1 * kph;
+        u = bu + 0.25 * ((double)sU[li] - bu) + a.c1 * ku;
+        v = bv + 0.25 * ((double)sV[li] - bv) + a.c1 * kv;
+      } else {  // kRK3S3: Yn + 2/3 (Y2 - Yn) + 2/3 dt k
+        ph = bph + (2.0 / 3.0) * ((double)a.in_phi[g] - bph) + a.c1 * kph;
+        u = bu + (2.0 / 3.0) * ((double)sU[li] - bu) + a.c1 * ku;
+        v = bv + (2.0 / 3.0) * ((double)sV[li] - bv) + a.c1 * kv;
+      }
+      const float hph = (float)ph, hu = (float)u, hv = (float)v;
+      a.o_phi[g] = hph;
+      a.o_u[g] = hu;
+      a.o_v[g] = hv;
+      if (a.oc_phi) {  // last stage: keep the rounding residue (Kahan word)
+        a.oc_phi[g] = (float)(ph - (double)hph);
+        a.oc_u[g] = (float)(u - (double)hu);
+        a.oc_v[g] = (float)(v - (double)hv);
+      }
+      if (i == 1 || i == npv || j == 1 || j == npv) {
+        const float vals[3] = {hph, hu, hv};
+        float lows[3] = {0.0f, 0.0f, 0.0f};
+        if (a.oc_phi) {
+          lows[0] = (float)(ph - (double)hph);
+          lows[1] = (float)(u - (double)hu);
+          lows[2] = (float)(v - (double)hv);
+        }
+#pragma unroll
+        for (int fl = 0; fl < 3; ++fl) {
+          long long sl[4];
+          int ns = 0;
+          if (i == 1) sl[ns++] = edge_slot(npv, p, 0, fl, j);
+          if (i == npv) sl[ns++] = edge_slot(npv, p, 1, fl, j);
+          if (j == 1) sl[ns++] = edge_slot(npv, p, 2, fl, i);
+          if (j == npv) sl[ns++] = edge_slot(npv, p, 3, fl, i);
+          for (int q = 0; q < ns; ++q) {
+            a.edge_raw[sl[q]] = vals[fl];
+            if (a.oc_phi) a.edge_raw_c[sl[q]] = lows[fl];
+          }
+        }
+      }
+    }
+  }
+}
+
+// correct_bdy_ghost on an FP32C level: same algorithm as the FP64 boundary
+// kernel, values in FP32 storage, arithmetic and ghost weights in FP64.  The
+// deviation phi' is interpolated and averaged like phi (both operations are
+// linear and phi_ref is already continuous across edges).  For the base state
+// (cphi != NULL) values are hi + lo and every written point gets its residue.
+template <int DOF>
+__global__ __launch_bounds__(256) void boundary_kernel_f32(MeshDev m, float *phi, float *u,
+                                                           float *v, float *cphi, float *cu,
+                                                           float *cv, const float *edge_raw,
+                                                           const float *edge_raw_c) {
+  constexpr int H = DOF - 1;
+  const int npv = m.npv, nh = m.nh;
+  const double dx = m.dx;
+  const int nside = 6 * 4 * npv;
+  auto raw = [&](int q, int i, int j, double &ph, double &uu, double &vv) {
+    int side = -1, s = 0;
+    if (i == 1) { side = 0; s = j; }
+    else if (i == npv) { side = 1; s = j; }
+    else if (j == 1) { side = 2; s = i; }
+    else if (j == npv) { side = 3; s = i; }
+    if (side >= 0) {
+      const long long s0 = edge_slot(npv, q, side, 0, s), s1 = edge_slot(npv, q, side, 1, s),
+                      s2 = edge_slot(npv, q, side, 2, s);
+      ph = edge_raw[s0]; uu = edge_raw[s1]; vv = edge_raw[s2];
+      if (cphi) {
+        ph += (double)edge_raw_c[s0]; uu += (double)edge_raw_c[s1]; vv += (double)edge_raw_c[s2];
+      }
+    } else {
+      const long long g = pidx(nh, H, q, i, j);
+      ph = phi[g]; uu = u[g]; vv = v[g];
+      if (cphi) { ph += (double)cphi[g]; uu += (double)cu[g]; vv += (double)cv[g]; }
+    }
+  };
+  auto store = [&](long long g, double ph, double uu, double vv) {
+    const float a0 = (float)ph, a1 = (float)uu, a2 = (float)vv;
+    phi[g] = a0; u[g] = a1; v[g] = a2;
+    if (cphi) {
+      cphi[g] = (float)(ph - (double)a0);
+      cu[g] = (float)(uu - (double)a1);
+      cv[g] = (float)(vv - (double)a2);
+    }
+  };
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nside + 8;
+       t += gridDim.x * blockDim.x) {
+    if (t < nside) {
+      const int s = t % npv + 1, ps = t / npv, side = ps % 4, p = ps / 4 + 1;
+      const HaloSrc hs = kHalo[p - 1][side];
+      for (int k = 1; k <= H; ++k) {
+        const int c = m.giref[(k - 1) * npv + (s - 1)];
+        const double cf = m.gcoef[(k - 1) * npv + (s - 1)];
+        const int nrm = hs.far ? npv - k : k + 1;
+        double qp[DOF], qz[DOF], qm[DOF];
+        for (int l = 0; l < DOF; ++l) {
+          const int tpos = (c - 1) * H + 1 + l;
+          const int al = hs.rev ? npv + 1 - tpos : tpos;
+          const int qi = hs.col ? nrm : al, qj = hs.col ? al : nrm;
+          double ph, uu, vv;
+          raw(hs.q, qi, qj, ph, uu, vv);
+          qp[l] = ph;
+          apply2(m.p2sp + 4 * pidx(nh, H, hs.q, qi, qj), uu, vv, qz[l], qm[l]);
+        }
+        int hi, hj;
+        if (side == 0) { hi = 1 - k; hj = s; }
+        else if (side == 1) { hi = npv + k; hj = s; }
+        else if (side == 2) { hi = s; hj = 1 - k; }
+        else { hi = s; hj = npv + k; }
+        const long long g = pidx(nh, H, p, hi, hj);
+        const double zh = ghost_interp<DOF>(qz, cf, dx), mh = ghost_interp<DOF>(qm, cf, dx);
+        double uh, vh;
+        apply2(m.sp2p + 4 * g, zh, mh, uh, vh);
+        store(g, ghost_interp<DOF>(qp, cf, dx), uh, vh);
+      }
+      const int e = kFirstEdge[p - 1][side];
+      if (e >= 0 && s > 1 && s < npv) {
+        const EdgePair ep = kEdges[e];
+        const int sb = ep.rev ? npv + 1 - s : s;
+        int ia, ja, ib, jb;
+        edge_ij(npv, ep.sa, s, ia, ja);
+        edge_ij(npv, ep.sb, sb, ib, jb);
+        const long long ga = pidx(nh, H, ep.pa, ia, ja), gb = pidx(nh, H, ep.pb, ib, jb);
+        double pa, ua, va, pb, ub, vb;
+        raw(ep.pa, ia, ja, pa, ua, va);
+        raw(ep.pb, ib, jb, pb, ub, vb);
+        double za, ma, zb, mb;
+        apply2(m.p2sp + 4 * ga, ua, va, za, ma);
+        apply2(m.p2sp + 4 * gb, ub, vb, zb, mb);
+        const double ph = 0.5 * (pa + pb), zu = 0.5 * (za + zb), mu = 0.5 * (ma + mb);
+        double u1, v1, u2, v2;
+        apply2(m.sp2p + 4 * ga, zu, mu, u1, v1);
+        apply2(m.sp2p + 4 * gb, zu, mu, u2, v2);
+        store(ga, ph, u1, v1);
+        store(gb, ph, u2, v2);
+      }
+    } else {
+      const int vx = t - nside;
+      double ph[3], z[3], mr[3];
+      long long gg[3];
+      for (int l = 0; l < 3; ++l) {
+        const VtxPt vp = kVerts[vx][l];
+        const int i = vp.ci ? npv : 1, j = vp.cj ? npv : 1;
+        double uu, vv;
+        raw(vp.p, i, j, ph[l], uu, vv);
+        gg[l] = pidx(nh, H, vp.p, i, j);
+        apply2(m.p2sp + 4 * gg[l], uu, vv, z[l], mr[l]);
+      }
+      const double pv = (ph[0] + ph[1] + ph[2]) / 3.0;
+      const double zv = (z[0] + z[1] + z[2]) / 3.0;
+      const double mv = (mr[0] + mr[1] + mr[2]) / 3.0;
+      for (int l = 0; l < 3; ++l) {
+        double uu, vv;
+        apply2(m.sp2p + 4 * gg[l], zv, mv, uu, vv);
+        store(gg[l], pv, uu, vv);
+      }
+    }
+  }
+}
+
+// pgref = RN32(phi_ref * sqrtG) on every point.
+__global__ void pgref_kernel(MeshDev m, const double *phi_ref, float *pgref) {
+  const long long per = (long long)m.nh * m.nh;
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < 6 * per;
+       g += (long long)gridDim.x * blockDim.x) {
+    const long long p = g / per;
+    pgref[g] = (float)(phi_ref[g] * m.sqrtG[p * m.metric_pstride + (g - p * per)]);
+  }
+}
+
+// FP64 edge-line copy of a finished level (to run the boundary pass on it).
+__global__ void edge_copy_kernel(MeshDev m, const double *phi, const double *u, const double *v,
+                                 double *edge_raw) {
+  const int npv = m.npv, H = m.dof - 1;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 6 * 4 * npv;
+       t += gridDim.x * blockDim.x) {
+    const int s = t % npv + 1, ps = t / npv, side = ps % 4, p = ps / 4 + 1;
+    int i, j;
+    edge_ij(npv, side, s, i, j);
+    const long long g = pidx(m.nh, H, p, i, j);
+    edge_raw[edge_slot(npv, p, side, 0, s)] = phi[g];
+    edge_raw[edge_slot(npv, p, side, 1, s)] = u[g];
+    edge_raw[edge_slot(npv, p, side, 2, s)] = v[g];
+  }
+}
+
+// FP64 MCV derivatives (x and y) of E_ref = phi_ref + phi_s with the
+// dissipation term left out: the part of dE/dx, dE/dy that does not depend on
+// the evolving state.  One thread per owned point; runs once per solve.
+template <int DOF>
+__global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, double *dEx,
+                                    double *dEy) {
+  constexpr int H = DOF - 1;
+  const int npv = m.npv, nh = m.nh;
+  const long long per = (long long)nh * nh;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < 6LL * npv * npv;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int p = (int)(t / ((long long)npv * npv)) + 1;
+    const int rem = (int)(t - (long long)(p - 1) * npv * npv);
+    const int j = rem / npv + 1, i = rem - (j - 1) * npv + 1;
+    auto E = [&](int ii, int jj) {
+      const long long g = pidx(nh, H, p, ii, jj);
+      return phi_ref[g] + m.phi_s[(long long)(p - 1) * m.phis_pstride + (g - (p - 1) * per)];
+    };
+    // derivative along one direction of the line through (i, j)
+    auto deriv = [&](bool xdir) {
+      const int c = xdir ? i : j;
+      const int pos = (c - 1) % H;
+      const int bL = c - pos;  // left cell end on the line
+      auto val = [&](int q) { return xdir ? E(q, j) : E(i, q); };
+      auto central = [&](int b) {  // 0.5*(polyfitR(left cell) + polyfitL(right cell)) at b
+        double l[DOF], r[DOF];
+        for (int k = 0; k < DOF; ++k) { l[k] = val(b - H + k); r[k] = val(b + k); }
+        return 0.5 * (fit_right<DOF>(l, m.dv) + fit_left<DOF>(r, m.dv));
+      };
+      const double dL = central(bL);
+      if (pos == 0) return dL;
+      double f[DOF];
+      for (int k = 0; k < DOF; ++k) f[k] = val(bL + k);
+      return cell_interior<DOF>(f, dL, central(bL + H), pos, m.dx, m.dv);
+    };
+    const long long g = pidx(nh, H, p, i, j);
+    dEx[g] = deriv(true);
+    dEy[g] = deriv(false);
+  }
+}
+
+// Splits the FP64 initial state into the FP32C state: phi' = phi - phi_ref
+// (zero in the interior, the halo-and-edge correction elsewhere), u and v,
+// each rounded to FP32 with its residue as the compensation word.
+__global__ void split_state_kernel(long long np, const double *phi, const double *phi_ref,
+                                   const double *u, const double *v, float *dphi, float *fu,
+                                   float *fv, float *cphi, float *cu, float *cv) {
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
+       g += (long long)gridDim.x * blockDim.x) {
+    const double d = phi[g] - phi_ref[g];
+    const float hd = (float)d, hu = (float)u[g], hv = (float)v[g];
+    dphi[g] = hd;
+    cphi[g] = (float)(d - (double)hd);
+    fu[g] = hu;
+    fv[g] = hv;
+    cu[g] = (float)(u[g] - (double)hu);
+    cv[g] = (float)(v[g] - (double)hv);
+  }
+}
+
+// Reconstructs the FP64 state of an FP32C level: phi = phi_ref + hi + lo,
+// u, v = hi + lo.
+__global__ void join_state_kernel(long long np, const double *phi_ref, const float *dphi,
+                                  const float *fu, const float *fv, const float *cphi,
+                                  const float *cu, const float *cv, double *phi, double *u,
+                                  double *v) {
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
+       g += (long long)gridDim.x * blockDim.x) {
+    phi[g] = phi_ref[g] + ((double)dphi[g] + (double)cphi[g]);
+    u[g] = (double)fu[g] + (double)cu[g];
+    v[g] = (double)fv[g] + (double)cv[g];
+  }
+}
+
+}  // namespace mcv

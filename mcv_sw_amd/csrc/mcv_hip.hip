@@ -14,6 +14,7 @@
 #include "../../include/mcv_hip.h"
 #include "mcv_device.hpp"
 #include "mcv_kernels_fp64.hpp"
+#include "mcv_fp32c.hpp"
 
 namespace mcv {
 
@@ -93,11 +94,21 @@ struct mcv_handle {
   double *diag_out = nullptr;
   double *pack[6] = {};
   int64_t steps = 0;
-  int tiles_x = 0;
+  int tile = 0;  // index into the tile shapes of the solve's order
   // per-kernel timing (mcv_hip_step_timed)
   bool timing = false;
   hipEvent_t ev[3] = {};
   double stage_ms = 0.0, boundary_ms = 0.0;
+  // FP32C form (precision == MCV_FP32C): FP32 levels of (phi - phi_ref, u, v),
+  // compensation words of Y_n, FP32 accumulator and edge copies; phi_ref is
+  // lv[0][0] (the projected initial geopotential, FP64).
+  bool fp32c = false;
+  float *lv32[3][3] = {};
+  float *comp32[3] = {};
+  float *acc32[3] = {};
+  float *edge32 = nullptr, *edge32c = nullptr;
+  double *join[3] = {};  // FP64 reconstruction of Y_n for read-back and integrals
+  MeshF32 mesh32{};
 };
 
 namespace {
@@ -149,23 +160,34 @@ int validate(const mcv_config *c) {
   if (c->scheme != MCV_RK4 && c->scheme != MCV_RK3_TVD)
     return fail(MCV_EINVAL, "Unknown integral scheme %d, select MCV_RK3_TVD or MCV_RK4",
                 c->scheme);
-  if (c->precision != MCV_FP64)
-    return fail(MCV_EINVAL, "precision %d is not available through this entry point",
-                c->precision);
+  if (c->precision != MCV_FP64 && c->precision != MCV_FP32C)
+    return fail(MCV_EINVAL, "precision %d: select MCV_FP64 or MCV_FP32C", c->precision);
   if (!(c->dt > 0.0) || !std::isfinite(c->dt)) return fail(MCV_EINVAL, "dt must be > 0");
   if (!(c->dx > 0.0) || !std::isfinite(c->dx)) return fail(MCV_EINVAL, "dx must be > 0");
   return MCV_OK;
 }
 
-template <int DOF>
-int launch_stage(mcv_handle *h, int kind, const StageArgs &a) {
-  constexpr int C = Tile<DOF>::C;
-  const int tx = (h->cfg.n + C - 1) / C;
-  dim3 grid(tx * tx * 6), block(kStageThreads);
+// Tile shapes (cells x cells per workgroup) the autotuner chooses from.
+constexpr int kTileVariants = 4;
+template <int DOF> struct TileSet;
+template <> struct TileSet<3> {
+  static constexpr int cx[kTileVariants] = {8, 16, 8, 12};
+  static constexpr int cy[kTileVariants] = {8, 4, 4, 6};
+};
+template <> struct TileSet<4> {
+  static constexpr int cx[kTileVariants] = {6, 8, 4, 10};
+  static constexpr int cy[kTileVariants] = {6, 4, 4, 4};
+};
+
+template <int DOF, int CX, int CY>
+int launch_stage_shape(mcv_handle *h, int kind, const StageArgs &a) {
+  const int tx = (h->cfg.n + CX - 1) / CX, ty = (h->cfg.n + CY - 1) / CY;
+  dim3 grid(tx * ty * 6), block(kStageThreads);
   switch (kind) {
-#define MCV_CASE(K)                                                                  \
-  case K:                                                                            \
-    hipLaunchKernelGGL((stage_kernel<DOF, K>), grid, block, 0, h->stream, h->mesh, a, tx); \
+#define MCV_CASE(K)                                                                   \
+  case K:                                                                             \
+    hipLaunchKernelGGL((stage_kernel<DOF, K, CX, CY>), grid, block, 0, h->stream, h->mesh, \
+                       a, tx);                                                        \
     break;
     MCV_CASE(kTend)
     MCV_CASE(kRK4S1)
@@ -181,6 +203,17 @@ int launch_stage(mcv_handle *h, int kind, const StageArgs &a) {
   }
   HIP_TRY(hipGetLastError());
   return MCV_OK;
+}
+
+template <int DOF>
+int launch_stage(mcv_handle *h, int kind, const StageArgs &a) {
+  using T = TileSet<DOF>;
+  switch (h->tile) {
+    case 0: return launch_stage_shape<DOF, T::cx[0], T::cy[0]>(h, kind, a);
+    case 1: return launch_stage_shape<DOF, T::cx[1], T::cy[1]>(h, kind, a);
+    case 2: return launch_stage_shape<DOF, T::cx[2], T::cy[2]>(h, kind, a);
+    default: return launch_stage_shape<DOF, T::cx[3], T::cy[3]>(h, kind, a);
+  }
 }
 
 int stage(mcv_handle *h, int kind, const StageArgs &a) {
@@ -199,6 +232,92 @@ int boundary(mcv_handle *h, int level) {
   return MCV_OK;
 }
 
+// ---------------------------------------------------------------------------
+// FP32C launches
+// ---------------------------------------------------------------------------
+template <int DOF> struct TileSet32;
+template <> struct TileSet32<3> {
+  static constexpr int cx[kTileVariants] = {8, 16, 16, 8};
+  static constexpr int cy[kTileVariants] = {8, 8, 4, 4};
+};
+template <> struct TileSet32<4> {
+  static constexpr int cx[kTileVariants] = {6, 8, 8, 10};
+  static constexpr int cy[kTileVariants] = {6, 4, 8, 6};
+};
+
+template <int DOF, int CX, int CY>
+int launch_stage32_shape(mcv_handle *h, int kind, const StageArgs32 &a) {
+  const int tx = (h->cfg.n + CX - 1) / CX, ty = (h->cfg.n + CY - 1) / CY;
+  dim3 grid(tx * ty * 6), block(256);
+  switch (kind) {
+#define MCV_CASE(K)                                                                       \
+  case K:                                                                                 \
+    hipLaunchKernelGGL((stage_kernel_f32<DOF, K, CX, CY>), grid, block, 0, h->stream,      \
+                       h->mesh32, a, tx, h->cfg.n);                                       \
+    break;
+    MCV_CASE(kRK4S1)
+    MCV_CASE(kRK4S2)
+    MCV_CASE(kRK4S3)
+    MCV_CASE(kRK4S4)
+    MCV_CASE(kRK3S1)
+    MCV_CASE(kRK3S2)
+    MCV_CASE(kRK3S3)
+#undef MCV_CASE
+    default:
+      return fail(MCV_EINVAL, "bad FP32C stage kind %d", kind);
+  }
+  HIP_TRY(hipGetLastError());
+  return MCV_OK;
+}
+
+template <int DOF>
+int launch_stage32(mcv_handle *h, int kind, const StageArgs32 &a) {
+  using T = TileSet32<DOF>;
+  switch (h->tile) {
+    case 0: return launch_stage32_shape<DOF, T::cx[0], T::cy[0]>(h, kind, a);
+    case 1: return launch_stage32_shape<DOF, T::cx[1], T::cy[1]>(h, kind, a);
+    case 2: return launch_stage32_shape<DOF, T::cx[2], T::cy[2]>(h, kind, a);
+    default: return launch_stage32_shape<DOF, T::cx[3], T::cy[3]>(h, kind, a);
+  }
+}
+
+int stage32(mcv_handle *h, int kind, const StageArgs32 &a) {
+  return h->cfg.dof == 3 ? launch_stage32<3>(h, kind, a) : launch_stage32<4>(h, kind, a);
+}
+
+int boundary32(mcv_handle *h, int level, bool base) {
+  float *c0 = base ? h->comp32[0] : nullptr, *c1 = base ? h->comp32[1] : nullptr,
+        *c2 = base ? h->comp32[2] : nullptr;
+  const int threads = 6 * 4 * h->npv + 8;
+  dim3 grid((threads + 255) / 256), block(256);
+  if (h->cfg.dof == 3)
+    hipLaunchKernelGGL(boundary_kernel_f32<3>, grid, block, 0, h->stream, h->mesh,
+                       h->lv32[level][0], h->lv32[level][1], h->lv32[level][2], c0, c1, c2,
+                       h->edge32, h->edge32c);
+  else
+    hipLaunchKernelGGL(boundary_kernel_f32<4>, grid, block, 0, h->stream, h->mesh,
+                       h->lv32[level][0], h->lv32[level][1], h->lv32[level][2], c0, c1, c2,
+                       h->edge32, h->edge32c);
+  HIP_TRY(hipGetLastError());
+  return MCV_OK;
+}
+
+StageArgs32 stage_args32(mcv_handle *h, int in, int out, double c1) {
+  StageArgs32 a{};
+  a.in_phi = h->lv32[in][0]; a.in_u = h->lv32[in][1]; a.in_v = h->lv32[in][2];
+  a.b_phi = h->lv32[0][0]; a.b_u = h->lv32[0][1]; a.b_v = h->lv32[0][2];
+  a.c_phi = h->comp32[0]; a.c_u = h->comp32[1]; a.c_v = h->comp32[2];
+  a.acc_phiG = h->acc32[0]; a.acc_u = h->acc32[1]; a.acc_v = h->acc32[2];
+  a.o_phi = h->lv32[out][0]; a.o_u = h->lv32[out][1]; a.o_v = h->lv32[out][2];
+  if (out == 0) {  // the step's last stage writes Y_n+1 with its compensation
+    a.oc_phi = h->comp32[0]; a.oc_u = h->comp32[1]; a.oc_v = h->comp32[2];
+  }
+  a.edge_raw = h->edge32;
+  a.edge_raw_c = h->edge32c;
+  a.c1 = c1;
+  return a;
+}
+
 // One RK stage: stage kernel from level `in` (base Yn = level 0) into level
 // `out`, then the boundary pass on `out`.
 int rk_stage(mcv_handle *h, int kind, int in, int out, double c1) {
@@ -211,11 +330,11 @@ int rk_stage(mcv_handle *h, int kind, int in, int out, double c1) {
   if (h->first_pending) { a.cu0 = h->cu0; a.cv0 = h->cv0; }
   a.c1 = c1;
   if (h->timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
-  int rc = stage(h, kind, a);
+  int rc = h->fp32c ? stage32(h, kind, stage_args32(h, in, out, c1)) : stage(h, kind, a);
   if (rc) return rc;
   h->first_pending = false;
   if (h->timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
-  rc = boundary(h, out);
+  rc = h->fp32c ? boundary32(h, out, out == 0) : boundary(h, out);
   if (rc) return rc;
   if (h->timing) {
     // events bracket each kernel on the handle's stream; wait so they can be reused
@@ -248,16 +367,34 @@ int one_step(mcv_handle *h) {
   return MCV_OK;
 }
 
+// FP64 (phi, u, v) of Y_n: the level itself, or for FP32C the reconstruction
+// phi_ref + hi + lo, hi + lo.
+int materialize(mcv_handle *h, const double **out) {
+  if (!h->fp32c) {
+    for (int f = 0; f < 3; ++f) out[f] = h->lv[0][f];
+    return MCV_OK;
+  }
+  hipLaunchKernelGGL(join_state_kernel, dim3(1024), dim3(256), 0, h->stream, h->np, h->lv[0][0],
+                     h->lv32[0][0], h->lv32[0][1], h->lv32[0][2], h->comp32[0], h->comp32[1],
+                     h->comp32[2], h->join[0], h->join[1], h->join[2]);
+  HIP_TRY(hipGetLastError());
+  for (int f = 0; f < 3; ++f) out[f] = h->join[f];
+  return MCV_OK;
+}
+
 int run_diag(mcv_handle *h, double *mass, double *energy) {
   const double *cu0 = h->first_pending ? h->cu0 : nullptr;
   const double *cv0 = h->first_pending ? h->cv0 : nullptr;
+  const double *y[3];
+  int rc = materialize(h, y);
+  if (rc) return rc;
   dim3 grid(h->diag_blocks), block(kDiagThreads);
   if (h->cfg.dof == 3)
-    hipLaunchKernelGGL(diag_partial_kernel<3>, grid, block, 0, h->stream, h->mesh, h->lv[0][0],
-                       h->lv[0][1], h->lv[0][2], cu0, cv0, h->partials);
+    hipLaunchKernelGGL(diag_partial_kernel<3>, grid, block, 0, h->stream, h->mesh, y[0], y[1],
+                       y[2], cu0, cv0, h->partials);
   else
-    hipLaunchKernelGGL(diag_partial_kernel<4>, grid, block, 0, h->stream, h->mesh, h->lv[0][0],
-                       h->lv[0][1], h->lv[0][2], cu0, cv0, h->partials);
+    hipLaunchKernelGGL(diag_partial_kernel<4>, grid, block, 0, h->stream, h->mesh, y[0], y[1],
+                       y[2], cu0, cv0, h->partials);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(1), 0, h->stream, h->partials,
                      h->diag_blocks, h->diag_out);
@@ -269,6 +406,94 @@ int run_diag(mcv_handle *h, double *mass, double *energy) {
   if (energy) *energy = r[1];
   if (!std::isfinite(r[0]) || !std::isfinite(r[1]))
     return fail(MCV_ENONFINITE, "non-finite integrals: mass %g energy %g", r[0], r[1]);
+  return MCV_OK;
+}
+
+template <typename T>
+int dev_alloc_t(mcv_handle *h, T **out, size_t count) {
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T) + 256);
+  if (e != hipSuccess)
+    return fail(MCV_ENOMEM, "hipMalloc(%zu bytes) failed: %s", count * sizeof(T),
+                hipGetErrorString(e));
+  h->allocs.push_back(p);
+  *out = static_cast<T *>(p);
+  return MCV_OK;
+}
+
+// FP32C set-up from the FP64 initial state in level 0.  One FP64 boundary
+// pass on a copy makes it continuous across panel edges with an interpolated
+// halo; that projected geopotential is the reference phi_ref (kept in level
+// 0).  The static FP64 gradient of phi_ref + phi_s follows, then the FP32
+// hi/lo split of the initial state itself (phi - phi_ref, u, v), so the first
+// stage sees the handed-over halo and edges as the FP64 path does.
+int setup32(mcv_handle *h) {
+  const long long np = h->np;
+  const size_t nedge = (size_t)6 * 4 * 3 * h->npv;
+  const int threads = 6 * 4 * h->npv;
+  int rc;
+  for (int f = 0; f < 3; ++f) {
+    if ((rc = dev_alloc(h, &h->join[f], np))) return rc;
+    HIP_TRY(hipMemcpyAsync(h->join[f], h->lv[0][f], np * sizeof(double),
+                           hipMemcpyDeviceToDevice, h->stream));
+  }
+  hipLaunchKernelGGL(edge_copy_kernel, dim3((threads + 255) / 256), dim3(256), 0, h->stream,
+                     h->mesh, h->lv[0][0], h->lv[0][1], h->lv[0][2], h->edge_raw);
+  HIP_TRY(hipGetLastError());
+  if ((rc = boundary(h, 0))) return rc;
+  h->first_pending = false;  // the FP32C stages form the contravariant winds themselves
+  double *dEx, *dEy;
+  if ((rc = dev_alloc(h, &dEx, np)) || (rc = dev_alloc(h, &dEy, np))) return rc;
+  if (h->cfg.dof == 3)
+    hipLaunchKernelGGL(ref_gradient_kernel<3>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
+                       h->lv[0][0], dEx, dEy);
+  else
+    hipLaunchKernelGGL(ref_gradient_kernel<4>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
+                       h->lv[0][0], dEx, dEy);
+  HIP_TRY(hipGetLastError());
+  for (int l = 0; l < 3; ++l)
+    for (int f = 0; f < 3; ++f)
+      if ((rc = dev_alloc_t(h, &h->lv32[l][f], np))) return rc;
+  for (int f = 0; f < 3; ++f) {
+    if ((rc = dev_alloc_t(h, &h->comp32[f], np))) return rc;
+    if ((rc = dev_alloc_t(h, &h->acc32[f], np))) return rc;
+  }
+  if ((rc = dev_alloc_t(h, &h->edge32, nedge)) || (rc = dev_alloc_t(h, &h->edge32c, nedge)))
+    return rc;
+  float *pgref;
+  if ((rc = dev_alloc_t(h, &pgref, np))) return rc;
+  hipLaunchKernelGGL(pgref_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh, h->lv[0][0],
+                     pgref);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(split_state_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->join[0],
+                     h->lv[0][0], h->join[1], h->join[2], h->lv32[0][0], h->lv32[0][1],
+                     h->lv32[0][2], h->comp32[0], h->comp32[1], h->comp32[2]);
+  HIP_TRY(hipGetLastError());
+  for (int l = 1; l < 3; ++l)
+    for (int f = 0; f < 3; ++f)
+      HIP_TRY(hipMemcpyAsync(h->lv32[l][f], h->lv32[0][f], np * sizeof(float),
+                             hipMemcpyDeviceToDevice, h->stream));
+  MeshF32 &m = h->mesh32;
+  const MeshDev &d = h->mesh;
+  m.dof = d.dof;
+  m.n = d.n;
+  m.npv = d.npv;
+  m.nh = d.nh;
+  m.rdx = (float)(1.0 / d.dx);
+  m.r2dx = (float)(1.0 / (2.0 * d.dx));
+  m.rdxx = (float)(1.0 / (d.dx * d.dx));
+  m.dxf = (float)d.dx;
+  m.metric_pstride = d.metric_pstride;
+  m.sg = d.sqrtG;
+  m.rsg = d.rsqrtG;
+  m.ig11 = d.ig11;
+  m.ig12 = d.ig12;
+  m.ig22 = d.ig22;
+  m.f = d.f;
+  m.pgref = pgref;
+  m.dEx = dEx;
+  m.dEy = dEy;
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return MCV_OK;
 }
 
@@ -341,6 +566,11 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   m.npv = h->npv;
   m.nh = h->nh;
   m.dx = cfg->dx;
+  m.dv.dx = make_recip(cfg->dx);
+  m.dv.dx2 = make_recip(2.0 * cfg->dx);
+  m.dv.dxx = make_recip(cfg->dx * cfg->dx);
+  std::vector<double> rsg(np);
+  for (long long g = 0; g < np; ++g) rsg[g] = 1.0 / mesh->sqrtG[g];
   const bool metric_inv = panel_invariant(mesh->sqrtG, per) && panel_invariant(ig11.data(), per) &&
                           panel_invariant(ig12.data(), per) && panel_invariant(ig22.data(), per);
   const long long mcount = metric_inv ? per : np;
@@ -350,6 +580,8 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   double *d;
   if ((rc = upload(h, &d, mesh->sqrtG, mcount))) return rc;
   m.sqrtG = d;
+  if ((rc = upload(h, &d, rsg.data(), mcount))) return rc;
+  m.rsqrtG = d;
   if ((rc = upload(h, &d, ig11.data(), mcount))) return rc;
   m.ig11 = d;
   if ((rc = upload(h, &d, ig12.data(), mcount))) return rc;
@@ -374,23 +606,27 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
                       hipMemcpyHostToDevice));
     m.giref = static_cast<const int32_t *>(pi);
   }
-  // state levels
-  for (int l = 0; l < 3; ++l)
+  // state levels (FP32C keeps only level 0 in FP64: the reference state)
+  h->fp32c = cfg->precision == MCV_FP32C;
+  const int nlev = h->fp32c ? 1 : 3;
+  for (int l = 0; l < nlev; ++l)
     for (int f = 0; f < 3; ++f)
       if ((rc = dev_alloc(h, &h->lv[l][f], np))) return rc;
-  for (int f = 0; f < 3; ++f)
-    if ((rc = dev_alloc(h, &h->acc[f], np))) return rc;
+  if (!h->fp32c)
+    for (int f = 0; f < 3; ++f)
+      if ((rc = dev_alloc(h, &h->acc[f], np))) return rc;
   if ((rc = dev_alloc(h, &h->edge_raw, (size_t)6 * 4 * 3 * h->npv))) return rc;
   HIP_TRY(hipMemcpy(h->lv[0][0], st->phi, np * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->lv[0][1], st->u, np * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->lv[0][2], st->v, np * sizeof(double), hipMemcpyHostToDevice));
   // halo corners of the scratch levels are never read, but keep them defined
-  for (int l = 1; l < 3; ++l)
+  for (int l = 1; l < nlev; ++l)
     for (int f = 0; f < 3; ++f)
       HIP_TRY(hipMemcpy(h->lv[l][f], h->lv[0][f], np * sizeof(double), hipMemcpyDeviceToDevice));
   if ((rc = upload(h, &h->cu0, st->contraU, np))) return rc;
   if ((rc = upload(h, &h->cv0, st->contraV, np))) return rc;
   h->first_pending = true;
+  if (h->fp32c && (rc = setup32(h))) return rc;
   // integrals
   const long long ncell = 6LL * cfg->n * cfg->n;
   h->diag_blocks = (int)std::min<long long>((ncell + kDiagThreads - 1) / kDiagThreads, 1024);
@@ -444,9 +680,12 @@ int mcv_hip_sync(mcv_handle *h, mcv_state *out, double *mass, double *energy) {
   if (out) {
     const double *cu0 = h->first_pending ? h->cu0 : nullptr;
     const double *cv0 = h->first_pending ? h->cv0 : nullptr;
-    hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh, h->lv[0][0],
-                       h->lv[0][1], h->lv[0][2], cu0, cv0, h->pack[0], h->pack[1], h->pack[2],
-                       h->pack[3], h->pack[4], h->pack[5]);
+    const double *y[3];
+    int rc = materialize(h, y);
+    if (rc) return rc;
+    hipLaunchKernelGGL(pack_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh, y[0], y[1],
+                       y[2], cu0, cv0, h->pack[0], h->pack[1], h->pack[2], h->pack[3],
+                       h->pack[4], h->pack[5]);
     HIP_TRY(hipGetLastError());
     double *dst[6] = {out->phi, out->phiG, out->u, out->v, out->contraU, out->contraV};
     for (int f = 0; f < 6; ++f)
@@ -463,8 +702,11 @@ int mcv_hip_sphere_winds(mcv_handle *h, double *zonal, double *meridional) {
   if (!h) return fail(MCV_EINVAL, "handle is NULL");
   if (!zonal || !meridional) return fail(MCV_EINVAL, "output array is NULL");
   HIP_TRY(hipSetDevice(h->cfg.device));
-  hipLaunchKernelGGL(sphere_winds_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh,
-                     h->lv[0][1], h->lv[0][2], h->pack[0], h->pack[1]);
+  const double *y[3];
+  int rc = materialize(h, y);
+  if (rc) return rc;
+  hipLaunchKernelGGL(sphere_winds_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh, y[1],
+                     y[2], h->pack[0], h->pack[1]);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(zonal, h->pack[0], h->np * sizeof(double), hipMemcpyDeviceToHost,
                          h->stream));
@@ -480,6 +722,8 @@ int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG, double 
   if (!in || !in->phi || !in->u || !in->v || !in->contraU || !in->contraV || !t_phiG || !t_u ||
       !t_v)
     return fail(MCV_EINVAL, "state or output array is NULL");
+  if (h->fp32c)
+    return fail(MCV_EINVAL, "the tendency seam evaluates the FP64 operator; this handle is FP32C");
   HIP_TRY(hipSetDevice(h->cfg.device));
   HIP_TRY(hipStreamSynchronize(h->stream));
   const size_t b = h->np * sizeof(double);
@@ -526,6 +770,55 @@ int mcv_hip_step_timed(mcv_handle *h, int32_t nsteps, double *stage_ms, double *
   if (rc) return rc;
   if (stage_ms) *stage_ms = h->stage_ms;
   if (boundary_ms) *boundary_ms = h->boundary_ms;
+  return MCV_OK;
+}
+
+int mcv_hip_autotune(mcv_handle *h, int32_t reps, int32_t *tile, double *ms_per_stage) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (reps < 1) return fail(MCV_EINVAL, "reps must be >= 1");
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  // A representative stage (reads Yn and the accumulator, writes S1 and the
+  // accumulator): S1, the accumulator and the raw edge copy are rewritten by
+  // the first stage of every step, so timing never changes the trajectory.
+  StageArgs a{};
+  a.in_phi = h->lv[0][0]; a.in_u = h->lv[0][1]; a.in_v = h->lv[0][2];
+  a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
+  a.acc_phiG = h->acc[0]; a.acc_u = h->acc[1]; a.acc_v = h->acc[2];
+  a.o_phi = h->lv[1][0]; a.o_u = h->lv[1][1]; a.o_v = h->lv[1][2];
+  a.edge_raw = h->edge_raw;
+  if (h->first_pending) { a.cu0 = h->cu0; a.cv0 = h->cv0; }
+  a.c1 = 0.5 * h->cfg.dt;
+  const int kind = h->cfg.scheme == MCV_RK4 ? kRK4S2 : kRK3S2;
+  const StageArgs32 a32 = h->fp32c ? stage_args32(h, 0, 1, 0.5 * h->cfg.dt) : StageArgs32{};
+  auto run = [&]() { return h->fp32c ? stage32(h, kind, a32) : stage(h, kind, a); };
+  const int keep = h->tile;
+  int best = keep;
+  double best_ms = 1e300;
+  for (int v = 0; v < kTileVariants; ++v) {
+    h->tile = v;
+    int rc = run();  // warm-up (code object load, caches)
+    if (rc) { h->tile = keep; return rc; }
+    HIP_TRY(hipEventRecord(h->ev[0], h->stream));
+    for (int r = 0; r < reps; ++r)
+      if ((rc = run())) { h->tile = keep; return rc; }
+    HIP_TRY(hipEventRecord(h->ev[1], h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+    if (ms_per_stage) ms_per_stage[v] = ms / reps;
+    if (ms < best_ms) { best_ms = ms; best = v; }
+  }
+  h->tile = best;
+  if (tile) *tile = best;
+  return MCV_OK;
+}
+
+int mcv_hip_set_tile(mcv_handle *h, int32_t tile) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (tile < 0 || tile >= kTileVariants)
+    return fail(MCV_EINVAL, "tile must be in [0, %d), got %d", kTileVariants, tile);
+  h->tile = tile;
   return MCV_OK;
 }
 

@@ -15,229 +15,9 @@
 #pragma once
 
 #include "mcv_device.hpp"
+#include "mcv_stage_fp64.hpp"
 
 namespace mcv {
-
-template <int DOF> struct Tile;
-template <> struct Tile<3> { static constexpr int C = 8; };  // 8x8 cells = 16x16 points
-template <> struct Tile<4> { static constexpr int C = 6; };  // 6x6 cells = 18x18 points
-
-constexpr int kStageThreads = 256;
-
-template <int DOF, int KIND>
-__global__ __launch_bounds__(kStageThreads) void stage_kernel(MeshDev m, StageArgs a,
-                                                              int tiles_x) {
-  constexpr int H = DOF - 1;
-  constexpr int C = Tile<DOF>::C;
-  constexpr int NR = C * H + 1;   // core extent incl. the far cell end
-  constexpr int TL = NR + 2 * H;  // staged extent incl. the halo cell
-  __shared__ double sE[TL * TL], sU[TL * TL], sV[TL * TL], sPG[TL * TL], sCU[TL * TL],
-      sCV[TL * TL];
-  __shared__ double lamx[(C + 1) * NR], lamy[(C + 1) * NR];
-  __shared__ double rxF[(C + 1) * NR], rxD[(C + 1) * NR], rxV[(C + 1) * NR];
-  __shared__ double ryF[(C + 1) * NR], ryD[(C + 1) * NR], ryU[(C + 1) * NR];
-
-  const int npv = m.npv, nh = m.nh, N = m.n;
-  const double dx = m.dx;
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int p = L % 6 + 1;
-  const int tile = L / 6;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
-  const int cx0 = tx * C, cy0 = ty * C;
-  const int ncx = min(C, N - cx0), ncy = min(C, N - cy0);
-  const int i0 = cx0 * H + 1, j0 = cy0 * H + 1;
-  const int nx = ncx * H + 1, ny = ncy * H + 1;           // core incl. far cell end
-  const int TXo = ncx * H + (cx0 + ncx == N ? 1 : 0);     // owned points
-  const int TYo = ncy * H + (cy0 + ncy == N ? 1 : 0);
-  const int lw = nx + 2 * H, lh = ny + 2 * H;
-  const long long pbase = (long long)(p - 1) * nh * nh;
-  const long long mbase = (long long)(p - 1) * m.metric_pstride;
-  const long long sbase = (long long)(p - 1) * m.phis_pstride;
-
-  // Phase 0: stage the cross-shaped tile + halo; derived fields per point.
-  for (int t = threadIdx.x; t < lw * lh; t += blockDim.x) {
-    const int ly = t / lw, lx = t - ly * lw;
-    const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
-    if (!inx && !iny) continue;  // halo corner: never read
-    const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
-    const long long g = pbase + off;
-    const double phi = a.in_phi[g], u = a.in_u[g], v = a.in_v[g];
-    const double sg = m.sqrtG[mbase + off];
-    const double g11 = m.ig11[mbase + off], g12 = m.ig12[mbase + off],
-                 g22 = m.ig22[mbase + off];
-    const double ps = m.phi_s[sbase + off];
-    double cu, cv;
-    if (a.cu0) {
-      cu = a.cu0[g];
-      cv = a.cv0[g];
-    } else {  // cov2contrav (projection_mod.F90:138-148)
-      cu = g11 * u + g12 * v;
-      cv = g12 * u + g22 * v;
-    }
-    const double K = 0.5 * (cu * u + cv * v);  // spatial_operators_mod.F90:60-61
-    const int li = ly * TL + lx;
-    sE[li] = phi + ps + K;
-    sU[li] = u;
-    sV[li] = v;
-    sPG[li] = phi * sg;
-    sCU[li] = cu;
-    sCV[li] = cv;
-    if (inx && iny) {
-      const int ox = lx - H, oy = ly - H;
-      if (ox % H == 0) lamx[(ox / H) * NR + oy] = wave_speed(cu, phi, g11);
-      if (oy % H == 0) lamy[(oy / H) * NR + ox] = wave_speed(cv, phi, g22);
-    }
-  }
-  __syncthreads();
-
-  // Phase 1: Riemann derivatives at every cell end of the core, x then y.
-  const int tasks_x = (ncx + 1) * ny, tasks_y = (ncy + 1) * nx;
-  for (int t = threadIdx.x; t < tasks_x + tasks_y; t += blockDim.x) {
-    const bool isx = t < tasks_x;
-    const int tt = isx ? t : t - tasks_x;
-    const int span = isx ? ny : nx;
-    const int mb = tt / span, o = tt - mb * span;
-    const int st = isx ? 1 : TL;  // stride along the sweep
-    const int b = isx ? (o + H) * TL + (H + mb * H) : (H + mb * H) * TL + (o + H);
-    const double *sC = isx ? sCU : sCV;
-    const double *sQ = isx ? sU : sV;   // q of the gradient flux
-    const double *sW = isx ? sV : sU;   // upwinded cross wind
-    double eL[DOF], eR[DOF], qL[DOF], qR[DOF], dL[DOF], dR[DOF], pL[DOF], pR[DOF], wL[DOF],
-        wR[DOF];
-#pragma unroll
-    for (int k = 0; k < DOF; ++k) {
-      const int l = b - H * st + k * st, r = b + k * st;
-      eL[k] = sE[l];  eR[k] = sE[r];
-      qL[k] = sQ[l];  qR[k] = sQ[r];
-      pL[k] = sPG[l]; pR[k] = sPG[r];
-      dL[k] = sPG[l] * sC[l];
-      dR[k] = sPG[r] * sC[r];
-      wL[k] = sW[l];  wR[k] = sW[r];
-    }
-    const double lam = isx ? lamx[mb * NR + o] : lamy[mb * NR + o];
-    const double up = upwind_sign(sC[b]);
-    const double fF = riemann(fit_right<DOF>(eL, dx), fit_left<DOF>(eR, dx),
-                              fit_right<DOF>(qL, dx), fit_left<DOF>(qR, dx), lam);
-    const double fD = riemann(fit_right<DOF>(dL, dx), fit_left<DOF>(dR, dx),
-                              fit_right<DOF>(pL, dx), fit_left<DOF>(pR, dx), lam);
-    const double wlr = fit_right<DOF>(wL, dx), wrl = fit_left<DOF>(wR, dx);
-    const double fW = riemann(wlr, wrl, wlr, wrl, up);
-    if (isx) {
-      rxF[mb * NR + o] = fF; rxD[mb * NR + o] = fD; rxV[mb * NR + o] = fW;
-    } else {
-      ryF[mb * NR + o] = fF; ryD[mb * NR + o] = fD; ryU[mb * NR + o] = fW;
-    }
-  }
-  __syncthreads();
-
-  // Phase 2: derivatives at owned points, tendencies, RK update.
-  for (int t = threadIdx.x; t < TXo * TYo; t += blockDim.x) {
-    const int oy = t / TXo, ox = t - oy * TXo;
-    const int lx = ox + H, ly = oy + H, li = ly * TL + lx;
-    double fx, dvx, vx, fy, dvy, uy;
-    {  // x sweep
-      const int mb = ox / H, pos = ox - mb * H;
-      if (pos == 0) {
-        fx = rxF[mb * NR + oy]; dvx = rxD[mb * NR + oy]; vx = rxV[mb * NR + oy];
-      } else {
-        const int b0 = ly * TL + H + mb * H;
-        double fe[DOF], fd[DOF], fv[DOF];
-#pragma unroll
-        for (int k = 0; k < DOF; ++k) {
-          fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
-        }
-        const int r0 = mb * NR + oy, r1 = (mb + 1) * NR + oy;
-        fx = cell_interior<DOF>(fe, rxF[r0], rxF[r1], pos, dx);
-        dvx = cell_interior<DOF>(fd, rxD[r0], rxD[r1], pos, dx);
-        vx = cell_interior<DOF>(fv, rxV[r0], rxV[r1], pos, dx);
-      }
-    }
-    {  // y sweep
-      const int mb = oy / H, pos = oy - mb * H;
-      if (pos == 0) {
-        fy = ryF[mb * NR + ox]; dvy = ryD[mb * NR + ox]; uy = ryU[mb * NR + ox];
-      } else {
-        const int b0 = (H + mb * H) * TL + lx;
-        double fe[DOF], fd[DOF], fu[DOF];
-#pragma unroll
-        for (int k = 0; k < DOF; ++k) {
-          const int l = b0 + k * TL;
-          fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
-        }
-        const int r0 = mb * NR + ox, r1 = (mb + 1) * NR + ox;
-        fy = cell_interior<DOF>(fe, ryF[r0], ryF[r1], pos, dx);
-        dvy = cell_interior<DOF>(fd, ryD[r0], ryD[r1], pos, dx);
-        uy = cell_interior<DOF>(fu, ryU[r0], ryU[r1], pos, dx);
-      }
-    }
-    const int i = i0 + ox, j = j0 + oy;
-    const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
-    const long long g = pbase + off;
-    const double sg = m.sqrtG[mbase + off];
-    const double fc = m.f[pbase + off];
-    const double cu = sCU[li], cv = sCV[li];
-    // spatial_operators_mod.F90:127-131
-    const double vort = (vx - uy) / sg;
-    const double kpg = -dvx - dvy;
-    const double ku = -fx + sg * (vort + fc) * cv;
-    const double kv = -fy - sg * (vort + fc) * cu;
-
-    if constexpr (KIND == kTend) {
-      const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
-      a.o_phi[ti] = kpg;
-      a.o_u[ti] = ku;
-      a.o_v[ti] = kv;
-      continue;
-    } else {
-      double pgn, un, vn;  // base state Y_n
-      if constexpr (KIND == kRK4S1 || KIND == kRK3S1) {
-        pgn = sPG[li]; un = sU[li]; vn = sV[li];
-      } else {
-        pgn = a.b_phi[g] * sg; un = a.b_u[g]; vn = a.b_v[g];
-      }
-      double pg, u, v;
-      if constexpr (KIND == kRK4S1) {
-        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-        a.acc_phiG[g] = kpg; a.acc_u[g] = ku; a.acc_v[g] = kv;
-      } else if constexpr (KIND == kRK4S2 || KIND == kRK4S3) {
-        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-        a.acc_phiG[g] = a.acc_phiG[g] + 2.0 * kpg;
-        a.acc_u[g] = a.acc_u[g] + 2.0 * ku;
-        a.acc_v[g] = a.acc_v[g] + 2.0 * kv;
-      } else if constexpr (KIND == kRK4S4) {
-        const double tpg = (a.acc_phiG[g] + kpg) / 6.0;
-        const double tu = (a.acc_u[g] + ku) / 6.0;
-        const double tv = (a.acc_v[g] + kv) / 6.0;
-        pg = pgn + a.c1 * tpg; u = un + a.c1 * tu; v = vn + a.c1 * tv;
-      } else if constexpr (KIND == kRK3S1) {
-        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-      } else if constexpr (KIND == kRK3S2) {
-        pg = 0.75 * pgn + 0.25 * sPG[li] + a.c1 * kpg;
-        u = 0.75 * un + 0.25 * sU[li] + a.c1 * ku;
-        v = 0.75 * vn + 0.25 * sV[li] + a.c1 * kv;
-      } else {  // kRK3S3
-        pg = pgn / 3.0 + 2.0 / 3.0 * sPG[li] + a.c1 * kpg;
-        u = un / 3.0 + 2.0 / 3.0 * sU[li] + a.c1 * ku;
-        v = vn / 3.0 + 2.0 / 3.0 * sV[li] + a.c1 * kv;
-      }
-      const double phi = pg / sg;  // time_scheme_mod.F90:81
-      a.o_phi[g] = phi;
-      a.o_u[g] = u;
-      a.o_v[g] = v;
-      // raw copy of edge-line points for the boundary pass
-      if (i == 1 || i == npv || j == 1 || j == npv) {
-        const double vals[3] = {phi, u, v};
-#pragma unroll
-        for (int fl = 0; fl < 3; ++fl) {
-          if (i == 1) a.edge_raw[edge_slot(npv, p, 0, fl, j)] = vals[fl];
-          if (i == npv) a.edge_raw[edge_slot(npv, p, 1, fl, j)] = vals[fl];
-          if (j == 1) a.edge_raw[edge_slot(npv, p, 2, fl, i)] = vals[fl];
-          if (j == npv) a.edge_raw[edge_slot(npv, p, 3, fl, i)] = vals[fl];
-        }
-      }
-    }
-  }
-}
 
 // Raw (pre-averaging) phi, u, v of interior point (q, i, j) after a stage:
 // edge-line points come from the raw copy, the rest from the level itself.

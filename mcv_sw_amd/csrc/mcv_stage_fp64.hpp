@@ -1,0 +1,267 @@
+// FP64 stage kernel: spatial operator + RK stage update on one (panel, tile).
+//
+// Phase 0 stages the cross-shaped tile plus its one-cell halo of (phi, u, v)
+// in LDS together with the derived E = phi + phi_s + K, phiG = phi*sqrtG and
+// the contravariant winds, and the wave speeds at cell ends.  Phase 1 computes
+// the Riemann derivative at every cell end of the tile once (the reference
+// computes each interface twice, from both cells).  Phase 2 forms the
+// interior-point derivatives, the vorticity/Coriolis tendencies and the RK
+// update, and writes the raw updated interior plus a raw copy of the panel's
+// edge lines for the boundary pass.  Every loop runs over compile-time
+// extents, so no index needs a run-time integer division.
+#pragma once
+
+#include "mcv_device.hpp"
+
+namespace mcv {
+
+constexpr int kStageThreads = 256;
+
+// Tile of CX x CY cells per workgroup (the autotuner picks among the shapes
+// instantiated in mcv_hip.hip).
+template <int DOF, int KIND, int CX, int CY>
+__global__ __launch_bounds__(kStageThreads) void stage_kernel(MeshDev m, StageArgs a,
+                                                              int tiles_x) {
+  constexpr int H = DOF - 1;
+  constexpr int NX = CX * H + 1;   // core extents incl. the far cell end
+  constexpr int NY = CY * H + 1;
+  constexpr int TX = NX + 2 * H;   // staged extents incl. the halo cell
+  constexpr int TY = NY + 2 * H;
+  __shared__ double sE[TX * TY], sU[TX * TY], sV[TX * TY], sPG[TX * TY], sCU[TX * TY],
+      sCV[TX * TY];
+  // x-sweep cell ends: (CX+1) per core row; y-sweep: (CY+1) per core column
+  __shared__ double lamx[(CX + 1) * NY], lamy[(CY + 1) * NX];
+  __shared__ double rxF[(CX + 1) * NY], rxD[(CX + 1) * NY], rxV[(CX + 1) * NY];
+  __shared__ double ryF[(CY + 1) * NX], ryD[(CY + 1) * NX], ryU[(CY + 1) * NX];
+
+  const int npv = m.npv, nh = m.nh, N = m.n;
+  const double dx = m.dx;
+  const MCVDivisors dv = m.dv;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int p = L % 6 + 1;
+  const int tile = L / 6;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  const int cx0 = tx * CX, cy0 = ty * CY;
+  const int ncx = min(CX, N - cx0), ncy = min(CY, N - cy0);
+  const int i0 = cx0 * H + 1, j0 = cy0 * H + 1;
+  const int nx = ncx * H + 1, ny = ncy * H + 1;           // core incl. far cell end
+  const int TXo = ncx * H + (cx0 + ncx == N ? 1 : 0);     // owned points
+  const int TYo = ncy * H + (cy0 + ncy == N ? 1 : 0);
+  const int lw = nx + 2 * H, lh = ny + 2 * H;
+  const long long pbase = (long long)(p - 1) * nh * nh;
+  const long long mbase = (long long)(p - 1) * m.metric_pstride;
+  const long long sbase = (long long)(p - 1) * m.phis_pstride;
+
+  // Phase 0: stage the cross-shaped tile + halo; derived fields per point.
+  for (int t = threadIdx.x; t < TX * TY; t += blockDim.x) {
+    const int ly = t / TX, lx = t - ly * TX;
+    if (lx >= lw || ly >= lh) continue;
+    const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
+    if (!inx && !iny) continue;  // halo corner: never read
+    const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
+    const long long g = pbase + off;
+    const double phi = a.in_phi[g], u = a.in_u[g], v = a.in_v[g];
+    const double sg = m.sqrtG[mbase + off];
+    const double g11 = m.ig11[mbase + off], g12 = m.ig12[mbase + off],
+                 g22 = m.ig22[mbase + off];
+    const double ps = m.phi_s[sbase + off];
+    double cu, cv;
+    if (a.cu0) {
+      cu = a.cu0[g];
+      cv = a.cv0[g];
+    } else {  // cov2contrav (projection_mod.F90:138-148)
+      cu = g11 * u + g12 * v;
+      cv = g12 * u + g22 * v;
+    }
+    const double K = 0.5 * (cu * u + cv * v);  // spatial_operators_mod.F90:60-61
+    const int li = ly * TX + lx;
+    sE[li] = phi + ps + K;
+    sU[li] = u;
+    sV[li] = v;
+    sPG[li] = phi * sg;
+    sCU[li] = cu;
+    sCV[li] = cv;
+    if (inx && iny) {
+      const int ox = lx - H, oy = ly - H;
+      if (ox % H == 0) lamx[(ox / H) * NY + oy] = wave_speed(cu, phi, g11);
+      if (oy % H == 0) lamy[(oy / H) * NX + ox] = wave_speed(cv, phi, g22);
+    }
+  }
+  __syncthreads();
+
+  // Phase 1: Riemann derivatives at every cell end of the core, x then y.
+  constexpr int kTasksX = (CX + 1) * NY, kTasksY = (CY + 1) * NX;
+  for (int t = threadIdx.x; t < kTasksX + kTasksY; t += blockDim.x) {
+    const bool isx = t < kTasksX;
+    const int tt = isx ? t : t - kTasksX;
+    int mb, o;
+    if (isx) { mb = tt / NY; o = tt - mb * NY; }
+    else { mb = tt / NX; o = tt - mb * NX; }
+    if (isx ? (mb > ncx || o >= ny) : (mb > ncy || o >= nx)) continue;
+    const int st = isx ? 1 : TX;  // stride along the sweep
+    const int b = isx ? (o + H) * TX + (H + mb * H) : (H + mb * H) * TX + (o + H);
+    const double *sC = isx ? sCU : sCV;
+    const double *sQ = isx ? sU : sV;   // q of the gradient flux
+    const double *sW = isx ? sV : sU;   // upwinded cross wind
+    double eL[DOF], eR[DOF], qL[DOF], qR[DOF], dL[DOF], dR[DOF], pL[DOF], pR[DOF], wL[DOF],
+        wR[DOF];
+#pragma unroll
+    for (int k = 0; k < DOF; ++k) {
+      const int l = b - H * st + k * st, r = b + k * st;
+      eL[k] = sE[l];  eR[k] = sE[r];
+      qL[k] = sQ[l];  qR[k] = sQ[r];
+      pL[k] = sPG[l]; pR[k] = sPG[r];
+      dL[k] = sPG[l] * sC[l];
+      dR[k] = sPG[r] * sC[r];
+      wL[k] = sW[l];  wR[k] = sW[r];
+    }
+    const double lam = isx ? lamx[tt] : lamy[tt];
+    const double up = upwind_sign(sC[b]);
+    const double fF = riemann(fit_right<DOF>(eL, dv), fit_left<DOF>(eR, dv),
+                              fit_right<DOF>(qL, dv), fit_left<DOF>(qR, dv), lam);
+    const double fD = riemann(fit_right<DOF>(dL, dv), fit_left<DOF>(dR, dv),
+                              fit_right<DOF>(pL, dv), fit_left<DOF>(pR, dv), lam);
+    const double wlr = fit_right<DOF>(wL, dv), wrl = fit_left<DOF>(wR, dv);
+    const double fW = riemann(wlr, wrl, wlr, wrl, up);
+    if (isx) {
+      rxF[tt] = fF; rxD[tt] = fD; rxV[tt] = fW;
+    } else {
+      ryF[tt] = fF; ryD[tt] = fD; ryU[tt] = fW;
+    }
+  }
+  __syncthreads();
+
+  // Phase 2: derivatives at owned points, tendencies, RK update.  Points are
+  // visited class by class (position within the cell in x and in y), cells
+  // fastest, so the lanes of a wave take the same branches and read LDS with
+  // a stride of H points (no bank conflicts); the far cell-end column and row
+  // owned by the last tiles follow.
+  constexpr int kCells = CX * CY, kSlots = H * H * kCells, kTail = NY + NX - 1;
+  for (int t = threadIdx.x; t < kSlots + kTail; t += blockDim.x) {
+    int ox, oy;
+    if (t < kSlots) {
+      const int cls = t / kCells, cell = t - cls * kCells;
+      const int cyl = cell / CX, cxl = cell - cyl * CX;
+      if (cxl >= ncx || cyl >= ncy) continue;
+      ox = cxl * H + cls % H;
+      oy = cyl * H + cls / H;
+    } else if (t < kSlots + NY) {
+      oy = t - kSlots;
+      ox = ncx * H;
+      if (ox >= TXo || oy >= TYo) continue;
+    } else {
+      ox = t - kSlots - NY;
+      oy = ncy * H;
+      if (ox >= ncx * H || oy >= TYo) continue;
+    }
+    const int lx = ox + H, ly = oy + H, li = ly * TX + lx;
+    double fx, dvx, vx, fy, dvy, uy;
+    {  // x sweep
+      const int mb = ox / H, pos = ox - mb * H;
+      const int r0 = mb * NY + oy;
+      if (pos == 0) {
+        fx = rxF[r0]; dvx = rxD[r0]; vx = rxV[r0];
+      } else {
+        const int b0 = ly * TX + H + mb * H;
+        double fe[DOF], fd[DOF], fv[DOF];
+#pragma unroll
+        for (int k = 0; k < DOF; ++k) {
+          fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
+        }
+        const int r1 = r0 + NY;
+        fx = cell_interior<DOF>(fe, rxF[r0], rxF[r1], pos, dx, dv);
+        dvx = cell_interior<DOF>(fd, rxD[r0], rxD[r1], pos, dx, dv);
+        vx = cell_interior<DOF>(fv, rxV[r0], rxV[r1], pos, dx, dv);
+      }
+    }
+    {  // y sweep
+      const int mb = oy / H, pos = oy - mb * H;
+      const int r0 = mb * NX + ox;
+      if (pos == 0) {
+        fy = ryF[r0]; dvy = ryD[r0]; uy = ryU[r0];
+      } else {
+        const int b0 = (H + mb * H) * TX + lx;
+        double fe[DOF], fd[DOF], fu[DOF];
+#pragma unroll
+        for (int k = 0; k < DOF; ++k) {
+          const int l = b0 + k * TX;
+          fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
+        }
+        const int r1 = r0 + NX;
+        fy = cell_interior<DOF>(fe, ryF[r0], ryF[r1], pos, dx, dv);
+        dvy = cell_interior<DOF>(fd, ryD[r0], ryD[r1], pos, dx, dv);
+        uy = cell_interior<DOF>(fu, ryU[r0], ryU[r1], pos, dx, dv);
+      }
+    }
+    const int i = i0 + ox, j = j0 + oy;
+    const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
+    const long long g = pbase + off;
+    const Recip sgr{m.sqrtG[mbase + off], m.rsqrtG[mbase + off]};
+    const double sg = sgr.b;
+    const double fc = m.f[pbase + off];
+    const double cu = sCU[li], cv = sCV[li];
+    // spatial_operators_mod.F90:127-131
+    const double vort = div_by(vx - uy, sgr);
+    const double kpg = -dvx - dvy;
+    const double ku = -fx + sg * (vort + fc) * cv;
+    const double kv = -fy - sg * (vort + fc) * cu;
+
+    if constexpr (KIND == kTend) {
+      const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
+      a.o_phi[ti] = kpg;
+      a.o_u[ti] = ku;
+      a.o_v[ti] = kv;
+    } else {
+      double pgn, un, vn;  // base state Y_n
+      if constexpr (KIND == kRK4S1 || KIND == kRK3S1) {
+        pgn = sPG[li]; un = sU[li]; vn = sV[li];
+      } else {
+        pgn = a.b_phi[g] * sg; un = a.b_u[g]; vn = a.b_v[g];
+      }
+      double pg, u, v;
+      if constexpr (KIND == kRK4S1) {
+        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
+        a.acc_phiG[g] = kpg; a.acc_u[g] = ku; a.acc_v[g] = kv;
+      } else if constexpr (KIND == kRK4S2 || KIND == kRK4S3) {
+        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
+        a.acc_phiG[g] = a.acc_phiG[g] + 2.0 * kpg;
+        a.acc_u[g] = a.acc_u[g] + 2.0 * ku;
+        a.acc_v[g] = a.acc_v[g] + 2.0 * kv;
+      } else if constexpr (KIND == kRK4S4) {
+        constexpr Recip k6{6.0, 1.0 / 6.0};
+        const double tpg = div_by(a.acc_phiG[g] + kpg, k6);
+        const double tu = div_by(a.acc_u[g] + ku, k6);
+        const double tv = div_by(a.acc_v[g] + kv, k6);
+        pg = pgn + a.c1 * tpg; u = un + a.c1 * tu; v = vn + a.c1 * tv;
+      } else if constexpr (KIND == kRK3S1) {
+        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
+      } else if constexpr (KIND == kRK3S2) {
+        pg = 0.75 * pgn + 0.25 * sPG[li] + a.c1 * kpg;
+        u = 0.75 * un + 0.25 * sU[li] + a.c1 * ku;
+        v = 0.75 * vn + 0.25 * sV[li] + a.c1 * kv;
+      } else {  // kRK3S3
+        constexpr Recip k3{3.0, 1.0 / 3.0};
+        pg = div_by(pgn, k3) + 2.0 / 3.0 * sPG[li] + a.c1 * kpg;
+        u = div_by(un, k3) + 2.0 / 3.0 * sU[li] + a.c1 * ku;
+        v = div_by(vn, k3) + 2.0 / 3.0 * sV[li] + a.c1 * kv;
+      }
+      const double phi = div_by(pg, sgr);  // time_scheme_mod.F90:81
+      a.o_phi[g] = phi;
+      a.o_u[g] = u;
+      a.o_v[g] = v;
+      // raw copy of edge-line points for the boundary pass
+      if (i == 1 || i == npv || j == 1 || j == npv) {
+        const double vals[3] = {phi, u, v};
+#pragma unroll
+        for (int fl = 0; fl < 3; ++fl) {
+          if (i == 1) a.edge_raw[edge_slot(npv, p, 0, fl, j)] = vals[fl];
+          if (i == npv) a.edge_raw[edge_slot(npv, p, 1, fl, j)] = vals[fl];
+          if (j == 1) a.edge_raw[edge_slot(npv, p, 2, fl, i)] = vals[fl];
+          if (j == npv) a.edge_raw[edge_slot(npv, p, 3, fl, i)] = vals[fl];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace mcv

@@ -7,7 +7,7 @@
 
 #include "../../include/mcv_hip.h"
 
-static int run(int dof, int n, int scheme) {
+static int run(int dof, int n, int scheme, int precision) {
   const int H = dof - 1, npv = n * H + 1, nh = npv + 2 * H;
   const size_t np = (size_t)6 * nh * nh;
   std::vector<double> sg(np), G(4 * np), IG(4 * np), A(4 * np), IA(4 * np), f(np), ps(np);
@@ -37,20 +37,27 @@ static int run(int dof, int n, int scheme) {
       iref[k * npv + s] = c;
       coef[k * npv + s] = 0.3 * dx;
     }
-  mcv_config cfg{MCV_ABI_VERSION, dof, n, scheme, MCV_FP64, 0, 1e-6, dx};
+  mcv_config cfg{MCV_ABI_VERSION, dof, n, scheme, precision, 0, 1e-6, dx};
   mcv_mesh mesh{sg.data(), G.data(), IG.data(), A.data(), IA.data(), f.data(), ps.data(),
                 coef.data(), iref.data()};
   mcv_state st{phi.data(), phiG.data(), u.data(), v.data(), cu.data(), cv.data()};
   mcv_handle *h = nullptr;
   if (mcv_hip_init(&cfg, &mesh, &st, &h)) { std::printf("init: %s\n", mcv_hip_last_error()); return 1; }
-  if (mcv_hip_step(h, 2)) { std::printf("step: %s\n", mcv_hip_last_error()); return 1; }
+  int32_t best = -1;
+  double tms[MCV_TILE_VARIANTS];
+  if (mcv_hip_autotune(h, 1, &best, tms)) { std::printf("autotune: %s\n", mcv_hip_last_error()); return 1; }
+  for (int v = 0; v < MCV_TILE_VARIANTS; ++v) {  // every tile shape through every stage kind
+    if (mcv_hip_set_tile(h, v)) return 1;
+    if (mcv_hip_step(h, 1)) { std::printf("step: %s\n", mcv_hip_last_error()); return 1; }
+  }
   double mass, energy;
   std::vector<double> z(np), m(np), t(3 * (size_t)6 * npv * npv);
   if (mcv_hip_sync(h, &st, &mass, &energy)) { std::printf("sync: %s\n", mcv_hip_last_error()); return 1; }
   if (mcv_hip_sphere_winds(h, z.data(), m.data())) { std::printf("winds: %s\n", mcv_hip_last_error()); return 1; }
-  if (mcv_hip_tendency(h, &st, t.data(), t.data() + 6 * npv * npv, t.data() + 12 * npv * npv)) { std::printf("tend: %s\n", mcv_hip_last_error()); return 1; }
+  const int trc = mcv_hip_tendency(h, &st, t.data(), t.data() + 6 * npv * npv, t.data() + 12 * npv * npv);
+  if (precision == MCV_FP64 ? trc != MCV_OK : trc != MCV_EINVAL) { std::printf("tend: %d %s\n", trc, mcv_hip_last_error()); return 1; }
   mcv_hip_free(h);
-  std::printf("dof %d n %d scheme %d ok mass %g\n", dof, n, scheme, mass);
+  std::printf("dof %d n %d scheme %d precision %d ok mass %g\n", dof, n, scheme, precision, mass);
   return 0;
 }
 
@@ -59,6 +66,7 @@ int main() {
   int rc = 0;
   for (int dof = 3; dof <= 4; ++dof)
     for (int n : {2, 5, 7, 13})
-      for (int scheme : {MCV_RK4, MCV_RK3_TVD}) rc |= run(dof, n, scheme);
+      for (int scheme : {MCV_RK4, MCV_RK3_TVD})
+        for (int precision : {MCV_FP64, MCV_FP32C}) rc |= run(dof, n, scheme, precision);
   return rc;
 }

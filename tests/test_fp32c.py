@@ -1,0 +1,126 @@
+"""The FP32-compensated (FP32C) form against the FP64 path.
+
+FP32C keeps (phi - phi_ref, u, v) in FP32 with an FP32 compensation word on
+the RK base state, runs the stencil in FP32 and assembles tendencies and RK
+updates in FP64.  It is not bit-identical to the reference by design; these
+tests bound its relative L2 distance from the FP64 trajectory and its mass
+drift, and check the properties it must keep exactly: the read-back of an
+unstepped state is the handed-over state, every tile shape gives the same
+bits, and the FP64-only tendency seam refuses an FP32C handle.
+
+Bounds (relative L2 over the owned points, h = phi / g):
+  h  <= 1e-7    u, v <= 5e-6    |mass drift| <= 1e-9
+after 20-40 steps at 6x12x12 and 6x16x16; the measured values are ~2e-8
+(h) and ~2e-7 .. 8e-7 (u, v) with a drift of ~1e-11 (the u, v norms are
+dominated by the smaller meridional parts, which is why their relative
+figures are larger).
+"""
+import numpy as np
+import pytest
+
+import cases
+from mcv_sw_amd import capi
+from oracle import oracle
+
+H_TOL, UV_TOL, MASS_TOL = 1e-7, 5e-6, 1e-9
+
+
+def _owned(a, dof, n):
+    h = dof - 1
+    npv = n * h + 1
+    nh = npv + 2 * h
+    return a.reshape(6, nh, nh)[:, h:h + npv, h:h + npv]
+
+
+def _rel(a, b, dof, n):
+    x, y = _owned(a, dof, n), _owned(b, dof, n)
+    return float(np.linalg.norm(y - x) / np.linalg.norm(x))
+
+
+def _check(ref, out, dof, n, mass0, what):
+    errs = {k: _rel(ref[k], out[k], dof, n) for k in ("phi", "u", "v")}
+    drift = abs(out["mass"] - mass0) / abs(mass0)
+    print(f"[fp32c] {what}: " + " ".join(f"{k} {v:.2e}" for k, v in errs.items())
+          + f" mass drift {drift:.2e}", flush=True)
+    assert errs["phi"] <= H_TOL, f"{what}: h error {errs['phi']:.3e}"
+    assert errs["u"] <= UV_TOL and errs["v"] <= UV_TOL, f"{what}: wind error {errs}"
+    assert drift <= MASS_TOL, f"{what}: mass drift {drift:.3e}"
+
+
+def _jet(dof, n):
+    m = cases.mesh(dof, n)
+    return m, cases.bumped_jet(m, u0=40.0, h0=8000.0, amp=60.0)
+
+
+# ---------------------------------------------------------------------------
+# CPU: the product sources compiled against the emulation shim
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("scheme", ["RK4", "RK3_TVD"])
+@pytest.mark.parametrize("dof,n,dt,steps", [(3, 12, 300.0, 30), (4, 12, 200.0, 30)])
+def test_emulated_fp32c_tracks_fp64(emu_lib, dof, n, dt, steps, scheme):
+    m, st = _jet(dof, n)
+    with capi.Solver(dof, n, dt, m, st, scheme=scheme, lib=emu_lib) as a, \
+            capi.Solver(dof, n, dt, m, st, scheme=scheme, precision=capi.FP32C,
+                        lib=emu_lib) as b:
+        mass0 = a.sync(fields=())["mass"]
+        a.step(steps)
+        b.step(steps)
+        _check(a.sync(), b.sync(), dof, n, mass0, f"emu dof={dof} n={n} {scheme}")
+
+
+@pytest.mark.parametrize("dof", [3, 4])
+def test_emulated_fp32c_readback_is_initial_state(emu_lib, dof):
+    n = 6
+    m, st = _jet(dof, n)
+    with capi.Solver(dof, n, 300.0, m, st, precision=capi.FP32C, lib=emu_lib) as s:
+        out = s.sync()
+    for k in ("phi", "u", "v"):
+        a, b = _owned(out[k], dof, n), _owned(st[k], dof, n)
+        # hi + lo holds 48 significant bits; phi is phi_ref + (hi + lo)
+        assert np.abs(a - b).max() <= 2.0 ** -46 * np.abs(b).max(), k
+
+
+def test_emulated_fp32c_refuses_tendency_seam(emu_lib):
+    m, st = _jet(3, 4)
+    with capi.Solver(3, 4, 300.0, m, st, precision=capi.FP32C, lib=emu_lib) as s:
+        with pytest.raises(capi.MCVError) as e:
+            s.tendency(st)
+    assert e.value.code == capi.MCV_EINVAL
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", ["RK4", "RK3_TVD"])
+@pytest.mark.parametrize("dof,n,dt", [(3, 16, 300.0), (4, 16, 150.0)])
+def test_fp32c_tracks_oracle(hip_lib, dof, n, dt, scheme):
+    print(f"[fp32c oracle] dof={dof} n={n} {scheme}", flush=True)
+    m, st = _jet(dof, n)
+    steps = 40
+    ref = oracle.run(dof, n, 4 if scheme == "RK4" else 3, dt, m, st, steps)
+    with capi.Solver(dof, n, dt, m, st, scheme=scheme) as a:
+        mass0 = a.sync(fields=())["mass"]
+    with capi.Solver(dof, n, dt, m, st, scheme=scheme, precision=capi.FP32C) as s:
+        s.step(steps)
+        out = s.sync()
+    _check(ref, out, dof, n, mass0, f"gpu dof={dof} n={n} {scheme}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dof,n", [(3, 13), (4, 11)])
+def test_fp32c_tile_shapes_identical(hip_lib, dof, n):
+    print(f"[fp32c tiles] dof={dof} n={n}", flush=True)
+    m, st = _jet(dof, n)
+    outs = []
+    for tile in range(capi.TILE_VARIANTS):
+        with capi.Solver(dof, n, 200.0, m, st, precision=capi.FP32C) as s:
+            s.set_tile(tile)
+            s.step(6)
+            outs.append(s.sync())
+    for tile in range(1, capi.TILE_VARIANTS):
+        for k in ("phi", "u", "v"):
+            assert np.array_equal(_owned(outs[tile][k], dof, n), _owned(outs[0][k], dof, n)), \
+                f"tile {tile} {k}"

@@ -134,3 +134,19 @@ def test_emulated_product_matches_reference(name, emu_lib):
         out = s.sync()
     assert state_digests(out, rec["dof"], rec["n"]) == rec["final"]
     assert abs(out["mass"] - rec["mass"]) <= 1e-14 * abs(rec["mass"])
+
+
+@pytest.mark.parametrize("tile", range(4))
+@pytest.mark.parametrize("name", ["tiny_mcv3_case5_rk4", "mcv4_n12_case2_rk4"])
+def test_emulated_tile_shapes_match_reference(name, tile, emu_lib):
+    """Every stage-kernel tile shape the autotuner may pick is bit-exact."""
+    from mcv_sw_amd import capi
+
+    rec = GOLD["cases"][name]
+    m, st = _ref_initial(name, rec)
+    with capi.Solver(rec["dof"], rec["n"], rec["dt"], m, st, scheme=rec["scheme"],
+                     lib=emu_lib) as s:
+        s.set_tile(tile)
+        s.step(rec["steps"])
+        out = s.sync()
+    assert state_digests(out, rec["dof"], rec["n"]) == rec["final"]

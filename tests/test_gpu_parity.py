@@ -118,3 +118,31 @@ def test_sphere_winds(hip_lib):
     msk = cases.masks(dof, n)["defined"]
     assert np.array_equal(z[:, msk], ez[:, msk])
     assert np.array_equal(w[:, msk], ew[:, msk])
+
+
+@pytest.mark.parametrize("tile", range(capi.TILE_VARIANTS))
+@pytest.mark.parametrize("dof,n", [(3, 13), (4, 11)])
+def test_tile_shapes_bitwise(hip_lib, dof, n, tile):
+    _say(f"[tile] dof={dof} n={n} tile={tile}")
+    m = cases.mesh(dof, n)
+    st = cases.case5(m)
+    ref = oracle.run(dof, n, 4, 300.0, m, st, 3)
+    with capi.Solver(dof, n, 300.0, m, st) as s:
+        s.set_tile(tile)
+        s.step(3)
+        out = s.sync()
+    _assert_bitwise(out, ref, dof, n, f"tile {tile}")
+
+
+def test_autotune_keeps_trajectory(hip_lib):
+    dof, n = 4, 12
+    m = cases.mesh(dof, n)
+    st = cases.case5(m)
+    ref = oracle.run(dof, n, 4, 300.0, m, st, 4)
+    with capi.Solver(dof, n, 300.0, m, st) as s:
+        s.step(1)
+        best, ms = s.autotune(reps=2)
+        assert 0 <= best < capi.TILE_VARIANTS and all(x > 0 for x in ms)
+        s.step(3)
+        out = s.sync()
+    _assert_bitwise(out, ref, dof, n, "after autotune")
