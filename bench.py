@@ -1,21 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark of the MCV_SW time step on one MI355X.
 
-Workload (N=1): the single large solve, MCV4 on the 6 x 384 x 384 gnomonic
-cubed sphere (~0.23 deg), RK4, FP64 drop-in, balanced jet (u0 = 40 m/s,
-h0 = 8000 m) plus a seeded (seed 0) degree-16 spherical-harmonic height
-perturbation at 60 m rms.  One step = one full RK4 time step of all 884,736
-cells; the state is resident in HBM before the timed region starts.
+Headline workload (value): the single large solve, MCV4 on the 6 x 384 x 384
+gnomonic cubed sphere (~0.23 deg), RK4, FP64 drop-in (bit-identical to the
+reference), balanced jet (u0 = 40 m/s, h0 = 8000 m) plus a seeded (seed 0)
+degree-16 spherical-harmonic height perturbation at 60 m rms, dt = 10 s.
+One step = one full RK4 time step of all 884,736 cells; the state is
+resident in HBM before the timed region starts.
 
-Prints one JSON line:
-  value        cell-steps/s of the timed steps (wall clock, stream synced on
-               both sides)
-  roofline     the stage kernel's achieved algorithmic HBM bandwidth (51 FP64
-               words per point per RK4 step, 9 points per MCV4 cell), from
-               HIP events on the solver's stream, against 8 TB/s
+Also reported on the same line:
+  fp32c        the FP32-compensated form on the same solve and step count:
+               cell-steps/s, and its h/u/v relative L2 and Linf distance and
+               mass drift from the FP64 trajectory after the same steps
+  batched      the uniform ensemble: 64 members of MCV3 on 6 x 90 x 90, RK3,
+               dt = 300 s, FP32C, member k perturbed with seed k (degree 12,
+               20 m rms) on the balanced jet u0 = 20 m/s, h0 = 5960 m; one
+               fused stage launch + one fused boundary launch per RK stage for
+               all members, replayed from a hipGraph
+  roofline     the FP64 stage kernel's achieved algorithmic HBM bandwidth
+               (51 FP64 words per point per RK4 step, 9 points per MCV4 cell)
+               from HIP events on the solver's stream, against 8 TB/s
   cpu_baseline the reference Fortran (oracle/_ref, built from the reference's
                own sources) on this host's cores, on a 2-step sample of the
-               same workload
+               headline workload
 Progress goes to stderr.
 """
 import argparse
@@ -26,6 +33,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -33,6 +42,10 @@ PEAK_HBM_GBS = 8000.0
 DOF, N, SCHEME = 4, 384, "RK4"
 DT = 10.0  # s; the configured 60 s is unstable for the reference scheme (Courant ~2.2)
 U0, H0, LMAX, SEED, RMS = 40.0, 8000.0, 16, 0, 60.0
+GRAVITY = 9.80616
+# uniform ensemble
+E_MEMBERS, E_DOF, E_N, E_DT, E_SCHEME = 64, 3, 90, 300.0, "RK3_TVD"
+E_U0, E_H0, E_LMAX, E_RMS = 20.0, 5960.0, 12, 20.0
 
 
 def log(msg):
@@ -40,8 +53,21 @@ def log(msg):
 
 
 def words_per_point(scheme):
-    # compulsory prognostic-state traffic per unique point per RK step
+    # compulsory FP64 prognostic-state traffic per unique point per RK step
     return 51 if scheme == "RK4" else 24
+
+
+def fp32c_bytes_per_point_step(scheme):
+    """Algorithmic HBM bytes per unique point per time step of the FP32C stage
+    kernels: FP32 stage input, base hi and lo, accumulator, output and (last
+    stage) output residue; FP32 phi_ref*sqrtG, FP64 reference gradient (2
+    words) and FP64 Coriolis parameter per stage."""
+    if scheme == "RK4":
+        # in 4x12, base hi 3x12, base lo 4x12, acc 12+24+24+12, out 4x12 + lo 12,
+        # static 4 x (4 + 16 + 8)
+        return 48 + 36 + 48 + 72 + 48 + 12 + 4 * 28
+    # RK3: in 3x12, base hi 2x12, base lo 3x12, out 3x12 + lo 12, static 3 x 28
+    return 36 + 24 + 36 + 36 + 12 + 3 * 28
 
 
 def measured_traffic(path):
@@ -55,6 +81,25 @@ def measured_traffic(path):
         return None
     m = re.search(r"Per RK4 step: (\d+(?:\.\d+)?) MB measured", text)
     return float(m.group(1)) * 1e6 / 4 if m else None
+
+
+def owned(a, dof, n):
+    h = dof - 1
+    npv = n * h + 1
+    nh = npv + 2 * h
+    return a.reshape(6, nh, nh)[:, h:h + npv, h:h + npv]
+
+
+def distance(ref, out, dof, n, mass0):
+    """Relative L2 and Linf distance of h, u, v, and the relative mass drift."""
+    r = {}
+    for k, name in (("phi", "h"), ("u", "u"), ("v", "v")):
+        x, y = owned(ref[k], dof, n), owned(out[k], dof, n)
+        r[f"{name}_rel_l2"] = float(np.linalg.norm(y - x) / np.linalg.norm(x))
+        r[f"{name}_rel_linf"] = float(np.abs(y - x).max() / np.abs(x).max())
+    r["mass_drift"] = float((out["mass"] - mass0) / mass0)
+    r["mass_drift_fp64"] = float((ref["mass"] - mass0) / mass0)
+    return r
 
 
 def cpu_baseline(mesh_fn, steps=2):
@@ -85,7 +130,6 @@ def cpu_baseline(mesh_fn, steps=2):
     from oracle import oracle
 
     om = dict(m)
-    om["dx"] = m["dx"]
     t = time.perf_counter()
     oracle.run(DOF, N, 4, DT, om, st, 1)
     sec = time.perf_counter() - t
@@ -94,12 +138,76 @@ def cpu_baseline(mesh_fn, steps=2):
                       f"restatement (gcc -O2, one core), {sec:.2f} s"}
 
 
+def run_single(capi, m, st, precision, steps, warmup):
+    """Times `steps` steps after warm-up and autotuning; returns the timing
+    record and the final state after warmup + 2 + steps + nt steps."""
+    cells = 6 * N * N
+    with capi.Solver(DOF, N, DT, m, st, scheme=SCHEME, precision=precision) as s:
+        mass0 = s.sync(fields=())["mass"]
+        s.step(warmup)
+        tile, tile_ms = s.autotune(reps=5)
+        s.step(2)
+        s.sync(fields=(), integrals=False)
+        t = time.perf_counter()
+        s.step(steps)
+        s.sync(fields=(), integrals=False)
+        wall = time.perf_counter() - t
+        nt = max(10, steps // 5)
+        stage_ms, bnd_ms = s.step_timed(nt)
+        out = s.sync(fields=("phi", "u", "v"))
+        out["mass0"] = mass0
+        rec = {"value": cells * steps / wall, "ms_per_step": wall * 1e3 / steps,
+               "stage_kernel_ms_per_step": stage_ms / nt,
+               "boundary_kernel_ms_per_step": bnd_ms / nt, "tile_variant": tile,
+               "tile_variant_ms_per_stage": tile_ms, "steps_compared": warmup + 2 + steps + nt}
+    return rec, out
+
+
+def run_ensemble(capi, hostlib, stages, warmup_stages):
+    """The uniform ensemble in one batch; cell-steps/s of all members."""
+    m = hostlib.mesh(E_DOF, E_N)
+    cells = E_MEMBERS * 6 * E_N * E_N
+    nst = 3  # RK3 stages per step
+    total_steps = (warmup_stages + stages + 2 * stages) // nst + 2
+    with capi.Batch(E_MEMBERS) as b:
+        t0 = time.time()
+        for k in range(E_MEMBERS):
+            st = hostlib.jet(m, u0=E_U0, h0=E_H0, lmax=E_LMAX, seed=k + 1, rms_h=E_RMS)
+            b.add(E_DOF, E_N, E_DT, m, st, total_steps, scheme=E_SCHEME, precision=capi.FP32C)
+        log(f"ensemble: {E_MEMBERS} members admitted in {time.time() - t0:.1f} s")
+        tile, tile_ms = b.autotune(reps=3)
+        b.run(warmup_stages)
+        b.poll()
+        t = time.perf_counter()
+        b.run(stages)
+        b.poll()
+        wall = time.perf_counter() - t
+        stage_ms, bnd_ms = b.run_timed(stages)
+        left, done = b.poll()
+    steps = stages / nst
+    bpps = fp32c_bytes_per_point_step(E_SCHEME)
+    pts = E_MEMBERS * 6 * (E_N * (E_DOF - 1) + 1) ** 2
+    achieved = bpps * pts * steps / (stage_ms / 1e3) / 1e9
+    return {"workload": f"{E_MEMBERS} members MCV3 6x{E_N}x{E_N}, RK3, dt={E_DT} s, FP32C, "
+                        f"one fused launch per stage for all members, hipGraph replay",
+            "value": cells * steps / wall, "unit": "cell-steps/s",
+            "ms_per_step": wall * 1e3 / steps, "stages_timed": stages,
+            "stage_kernel_ms_per_step": stage_ms / steps,
+            "boundary_kernel_ms_per_step": bnd_ms / steps,
+            "stage_kernel_GBs": achieved, "stage_kernel_frac": achieved / PEAK_HBM_GBS,
+            "algorithmic_bytes_per_point_step": bpps,
+            "tile_variant": tile, "tile_variant_ms_per_stage": tile_ms,
+            "members_done_steps": int(done[:E_MEMBERS].min())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32c", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -122,36 +230,21 @@ def main():
     st = hostlib.jet(m, u0=U0, h0=H0, lmax=LMAX, seed=SEED, rms_h=RMS)
     log(f"mesh + initial state: {time.time() - t0:.1f} s")
     cells = 6 * N * N
-    with capi.Solver(DOF, N, DT, m, st, scheme=SCHEME) as s:
-        del st
-        log("solver initialised")
-        s.step(args.warmup)
-        tile, tile_ms = s.autotune(reps=5)
-        log(f"autotuned tile shape {tile}: " + ", ".join(f"{x:.3f}" for x in tile_ms) + " ms")
-        s.step(2)
-        s.sync(fields=(), integrals=False)
-        t = time.perf_counter()
-        s.step(args.steps)
-        s.sync(fields=(), integrals=False)
-        wall = time.perf_counter() - t
-        log(f"timed {args.steps} steps: {wall * 1e3 / args.steps:.3f} ms/step")
-        nt = max(10, args.steps // 5)
-        stage_ms, bnd_ms = s.step_timed(nt)
-        out = s.sync(fields=())
-    value = cells * args.steps / wall
+    rec64, out64 = run_single(capi, m, st, capi.FP64, args.steps, args.warmup)
+    log(f"FP64: {rec64['ms_per_step']:.3f} ms/step")
+    nt_launches = 4 * max(10, args.steps // 5)
     bpcs = words_per_point(SCHEME) * 8 * (DOF - 1) ** 2  # bytes per cell-step
-    launches = 4 * nt
     bytes_per_launch = bpcs * cells / 4
-    avg_launch_s = stage_ms / 1e3 / launches
+    avg_launch_s = rec64["stage_kernel_ms_per_step"] / 4 / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
     res = {
         "metric": "SW cell-steps/s (plain FP64 vs batched FP32) at stated h err; serve p50/p99",
-        "value": value,
+        "value": rec64["value"],
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": rec64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -169,12 +262,30 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "kernel": "stage_kernel (4 launches per RK4 step)",
                      "bytes_per_cell_step": bpcs,
-                     "stage_kernel_ms_per_step": stage_ms / nt,
-                     "tile_variant": tile,
-                     "tile_variant_ms_per_stage": tile_ms,
-                     "boundary_kernel_ms_per_step": bnd_ms / nt},
-        "mass": out["mass"],
+                     "launches_timed": nt_launches,
+                     "stage_kernel_ms_per_step": rec64["stage_kernel_ms_per_step"],
+                     "tile_variant": rec64["tile_variant"],
+                     "tile_variant_ms_per_stage": rec64["tile_variant_ms_per_stage"],
+                     "boundary_kernel_ms_per_step": rec64["boundary_kernel_ms_per_step"]},
+        "mass_drift_fp64": float((out64["mass"] - out64["mass0"]) / out64["mass0"]),
     }
+    if not args.no_fp32c:
+        rec32, out32 = run_single(capi, m, st, capi.FP32C, args.steps, args.warmup)
+        log(f"FP32C: {rec32['ms_per_step']:.3f} ms/step")
+        err = distance(out64, out32, DOF, N, out32["mass0"])
+        bpp = fp32c_bytes_per_point_step(SCHEME)
+        pts = 6 * (N * (DOF - 1) + 1) ** 2
+        ach32 = bpp * pts / (rec32["stage_kernel_ms_per_step"] / 1e3) / 1e9
+        rec32.update({"speedup_vs_fp64": rec32["value"] / rec64["value"],
+                      "error_vs_fp64": err,
+                      "stage_kernel_GBs": ach32, "stage_kernel_frac": ach32 / PEAK_HBM_GBS,
+                      "algorithmic_bytes_per_point_step": bpp})
+        res["fp32c"] = rec32
+        del out32
+    del out64
+    if not args.no_batch:
+        res["batched"] = run_ensemble(capi, hostlib, stages=3 * 60, warmup_stages=3 * 4)
+        log(f"ensemble: {res['batched']['value']:.3e} cell-steps/s")
     cp_ms = capi.copy_probe(1 << 30, reps=10)
     res["roofline"]["copy_kernel_GBs"] = 2 * (1 << 30) / (cp_ms / 1e3) / 1e9
     if not args.no_cpu_baseline:

@@ -141,6 +141,72 @@ int mcv_hip_copy_probe(int32_t device, int64_t bytes, int32_t reps, double *ms_p
 /* Releases the handle and its device memory.  NULL is accepted. */
 int mcv_hip_free(mcv_handle *h);
 
+/* Device bytes a solve of this configuration allocates at most (no
+ * counterpart in the reference; the admission test of the batch API). */
+int mcv_hip_footprint(const mcv_config *cfg, int64_t *bytes);
+
+/* Device bytes the handle holds. */
+int mcv_hip_device_bytes(const mcv_handle *h, int64_t *bytes);
+
+/* ---------------------------------------------------------------------------
+ * Batches (no counterpart in the reference, which runs one solve per
+ * process: MCV_SW.F90:48-77).  A batch holds up to `capacity` solves of any
+ * order, resolution, precision, scheme and time step on one device.  One
+ * mcv_batch_run "stage" advances every solve in the batch by one RK stage of
+ * its own scheme with one fused stage launch and one fused boundary launch;
+ * solves admitted at different times run interleaved, each at its own
+ * stage.  Each solve advances by exactly the nsteps given at admission and
+ * then idles until removed.  Results are bit-identical to the same solve run
+ * alone through mcv_hip_step.  One host thread per batch.
+ * ------------------------------------------------------------------------- */
+typedef struct mcv_batch mcv_batch;
+
+int mcv_batch_create(int32_t device, int32_t capacity, mcv_batch **out);
+
+/* Admits a solve (uploads mesh and initial state exactly as mcv_hip_init
+ * does) that will advance nsteps time steps; returns its slot.  Returns
+ * MCV_ESTATE when every slot is taken. */
+int mcv_batch_add(mcv_batch *b, const mcv_config *cfg, const mcv_mesh *mesh,
+                  const mcv_state *initial, int64_t nsteps, int32_t *slot);
+
+/* Retires a solve and releases its device memory (blocks). */
+int mcv_batch_remove(mcv_batch *b, int32_t slot);
+
+/* Enqueues nstages stage launches (replays of a captured hipGraph where
+ * possible) and returns. */
+int mcv_batch_run(mcv_batch *b, int32_t nstages);
+
+/* Blocks, then writes for every slot (capacity entries, either pointer may
+ * be NULL) the time steps still to run and those completed, -1 for free
+ * slots. */
+int mcv_batch_poll(mcv_batch *b, int64_t *steps_left, int64_t *steps_done);
+
+/* mcv_hip_sync for one slot; MCV_ESTATE while the solve is inside a step. */
+int mcv_batch_sync(mcv_batch *b, int32_t slot, mcv_state *out, double *mass,
+                   double *energy);
+
+/* The slot's solve as a handle (owned by the batch: never step or free it). */
+int mcv_batch_handle(mcv_batch *b, int32_t slot, mcv_handle **h);
+
+/* Times the MCV_TILE_VARIANTS tile shapes on the batch's current solves
+ * (reps stage launches each, trajectory unchanged; every solve must be at a
+ * step boundary) and keeps the fastest.  Blocks. */
+int mcv_batch_autotune(mcv_batch *b, int32_t reps, int32_t *tile, double *ms_per_stage);
+
+int mcv_batch_set_tile(mcv_batch *b, int32_t tile);
+
+/* Graph replay on (default) or off: off launches every kernel directly. */
+int mcv_batch_set_graphs(mcv_batch *b, int32_t on);
+
+/* Measurement aid: like mcv_batch_run without graphs, with HIP events around
+ * every kernel; blocks and returns the summed stage-kernel and
+ * boundary-kernel milliseconds. */
+int mcv_batch_run_timed(mcv_batch *b, int32_t nstages, double *stage_ms,
+                        double *boundary_ms);
+
+/* Releases the batch and every solve in it.  NULL is accepted. */
+int mcv_batch_free(mcv_batch *b);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *mcv_hip_last_error(void);
 

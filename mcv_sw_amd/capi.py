@@ -87,8 +87,30 @@ def load(path=None):
     L.mcv_hip_free.argtypes = [H]
     L.mcv_hip_last_error.argtypes = []
     L.mcv_hip_last_error.restype = ctypes.c_char_p
+    L.mcv_hip_footprint.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_int64)]
+    L.mcv_hip_device_bytes.argtypes = [H, ctypes.POINTER(ctypes.c_int64)]
+    B = ctypes.c_void_p
+    _L64 = ctypes.POINTER(ctypes.c_int64)
+    L.mcv_batch_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(B)]
+    L.mcv_batch_add.argtypes = [B, ctypes.POINTER(Config), ctypes.POINTER(Mesh),
+                                ctypes.POINTER(State), ctypes.c_int64,
+                                ctypes.POINTER(ctypes.c_int32)]
+    L.mcv_batch_remove.argtypes = [B, ctypes.c_int32]
+    L.mcv_batch_run.argtypes = [B, ctypes.c_int32]
+    L.mcv_batch_poll.argtypes = [B, _L64, _L64]
+    L.mcv_batch_sync.argtypes = [B, ctypes.c_int32, ctypes.POINTER(State), _D, _D]
+    L.mcv_batch_handle.argtypes = [B, ctypes.c_int32, ctypes.POINTER(H)]
+    L.mcv_batch_autotune.argtypes = [B, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _D]
+    L.mcv_batch_set_tile.argtypes = [B, ctypes.c_int32]
+    L.mcv_batch_set_graphs.argtypes = [B, ctypes.c_int32]
+    L.mcv_batch_run_timed.argtypes = [B, ctypes.c_int32, _D, _D]
+    L.mcv_batch_free.argtypes = [B]
     for f in ("mcv_hip_init", "mcv_hip_step", "mcv_hip_sync", "mcv_hip_sphere_winds",
-              "mcv_hip_tendency", "mcv_hip_steps_done", "mcv_hip_step_timed", "mcv_hip_free"):
+              "mcv_hip_tendency", "mcv_hip_steps_done", "mcv_hip_step_timed", "mcv_hip_free",
+              "mcv_hip_footprint", "mcv_hip_device_bytes", "mcv_batch_create", "mcv_batch_add",
+              "mcv_batch_remove", "mcv_batch_run", "mcv_batch_poll", "mcv_batch_sync",
+              "mcv_batch_handle", "mcv_batch_autotune", "mcv_batch_set_tile",
+              "mcv_batch_set_graphs", "mcv_batch_run_timed", "mcv_batch_free"):
         getattr(L, f).restype = ctypes.c_int
     _libs[path] = L
     return L
@@ -115,6 +137,49 @@ def _ptr(a, dtype=np.float64):
     return a.ctypes.data_as(_I if dtype == np.int32 else _D)
 
 
+def _inputs(dof, n, dt, mesh, state, scheme, precision, device):
+    """ctypes structs of one solve's configuration, mesh and initial state; the
+    returned list keeps the contiguous host arrays alive for the call."""
+    keep = []
+
+    def arr(a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return _ptr(a, dtype)
+
+    cfg = Config(ABI_VERSION, dof, n, SCHEMES.get(scheme, scheme), precision, device,
+                 float(dt), float(mesh["dx"]))
+    m = Mesh(arr(mesh["sqrtG"]), arr(mesh["G"]), arr(mesh["IG"]), arr(mesh["A"]),
+             arr(mesh["IA"]), arr(mesh["f"]), arr(mesh["phi_s"]), arr(mesh["coef"]),
+             arr(mesh["iref"], np.int32))
+    s = State(*[arr(state[k]) for k in STATE_FIELDS])
+    return cfg, m, s, keep
+
+
+def _read_state(call, nh, fields, integrals):
+    """Runs a sync-like entry point `call(state*, mass*, energy*)`."""
+    out = {k: np.empty((6, nh, nh)) for k in fields}
+    st = State(*[_ptr(out[k]) if k in out else None for k in STATE_FIELDS])
+    mass, energy = ctypes.c_double(), ctypes.c_double()
+    rc = call(ctypes.byref(st), ctypes.byref(mass) if integrals else None,
+              ctypes.byref(energy) if integrals else None)
+    if integrals:
+        out["mass"], out["energy"] = mass.value, energy.value
+    return rc, out
+
+
+def footprint(dof, n, precision=FP64, scheme="RK4", device=0, lib=None):
+    """Device bytes one solve of this configuration allocates at most."""
+    L = load(lib)
+    cfg = Config(ABI_VERSION, dof, n, SCHEMES.get(scheme, scheme), precision, device, 1.0,
+                 (np.pi / 2) / n)
+    b = ctypes.c_int64()
+    rc = L.mcv_hip_footprint(ctypes.byref(cfg), ctypes.byref(b))
+    if rc != MCV_OK:
+        raise MCVError(rc, L.mcv_hip_last_error().decode())
+    return b.value
+
+
 class Solver:
     """One cubed-sphere solve held on the GPU (an mcv_handle)."""
 
@@ -123,23 +188,11 @@ class Solver:
         self.lib = load(lib)
         self.dof, self.n = dof, n
         self.h, self.npv, self.nh = dims(dof, n)
-        cfg = Config(ABI_VERSION, dof, n, SCHEMES.get(scheme, scheme), precision, device,
-                     float(dt), float(mesh["dx"]))
-        self._keep = []
-
-        def arr(a, dtype=np.float64):
-            a = np.ascontiguousarray(a, dtype=dtype)
-            self._keep.append(a)
-            return _ptr(a, dtype)
-
-        m = Mesh(arr(mesh["sqrtG"]), arr(mesh["G"]), arr(mesh["IG"]), arr(mesh["A"]),
-                 arr(mesh["IA"]), arr(mesh["f"]), arr(mesh["phi_s"]), arr(mesh["coef"]),
-                 arr(mesh["iref"], np.int32))
-        s = State(*[arr(state[k]) for k in STATE_FIELDS])
+        cfg, m, s, keep = _inputs(dof, n, dt, mesh, state, scheme, precision, device)
         self._h = ctypes.c_void_p()
         self._check(self.lib.mcv_hip_init(ctypes.byref(cfg), ctypes.byref(m), ctypes.byref(s),
                                           ctypes.byref(self._h)))
-        self._keep = []
+        del keep
 
     def _check(self, rc):
         if rc != MCV_OK:
@@ -150,16 +203,16 @@ class Solver:
 
     def sync(self, fields=STATE_FIELDS, integrals=True):
         """Blocking read-back: dict of fields (6, nh, nh) plus 'mass', 'energy'."""
-        out = {k: np.empty((6, self.nh, self.nh)) for k in fields}
-        st = State(*[_ptr(out[k]) if k in out else None for k in STATE_FIELDS])
-        mass, energy = ctypes.c_double(), ctypes.c_double()
-        rc = self.lib.mcv_hip_sync(self._h, ctypes.byref(st),
-                                   ctypes.byref(mass) if integrals else None,
-                                   ctypes.byref(energy) if integrals else None)
-        if integrals:
-            out["mass"], out["energy"] = mass.value, energy.value
+        rc, out = _read_state(lambda st, m, e: self.lib.mcv_hip_sync(self._h, st, m, e),
+                              self.nh, fields, integrals)
         self._check(rc)
         return out
+
+    @property
+    def device_bytes(self):
+        b = ctypes.c_int64()
+        self._check(self.lib.mcv_hip_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
 
     def sphere_winds(self):
         z = np.empty((6, self.nh, self.nh))
@@ -203,6 +256,96 @@ class Solver:
         if self._h:
             self.lib.mcv_hip_free(self._h)
             self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Solves of mixed order, size, precision and scheme advanced together on
+    one GPU (an mcv_batch): one fused stage launch and one fused boundary
+    launch per RK stage for all of them, replayed from a captured hipGraph."""
+
+    GRAPH_STAGES = 12
+
+    def __init__(self, capacity, device=0, lib=None):
+        self.lib = load(lib)
+        self.capacity = capacity
+        self._b = ctypes.c_void_p()
+        self.info = {}
+        self._check(self.lib.mcv_batch_create(int(device), int(capacity), ctypes.byref(self._b)))
+        self.device = device
+
+    def _check(self, rc):
+        if rc != MCV_OK:
+            raise MCVError(rc, self.lib.mcv_hip_last_error().decode())
+
+    def add(self, dof, n, dt, mesh, state, nsteps, scheme="RK4", precision=FP64):
+        """Admit a solve that will run nsteps time steps; returns its slot."""
+        cfg, m, s, keep = _inputs(dof, n, dt, mesh, state, scheme, precision, self.device)
+        slot = ctypes.c_int32()
+        self._check(self.lib.mcv_batch_add(self._b, ctypes.byref(cfg), ctypes.byref(m),
+                                           ctypes.byref(s), int(nsteps), ctypes.byref(slot)))
+        del keep
+        self.info[slot.value] = dict(dof=dof, n=n, dt=dt, scheme=scheme, precision=precision,
+                                     nsteps=nsteps)
+        return slot.value
+
+    def remove(self, slot):
+        self._check(self.lib.mcv_batch_remove(self._b, int(slot)))
+        self.info.pop(slot, None)
+
+    def run(self, nstages):
+        """Enqueue nstages stage launches (each advances every solve one stage)."""
+        self._check(self.lib.mcv_batch_run(self._b, int(nstages)))
+
+    def poll(self):
+        """(steps_left, steps_done) per slot, -1 for free slots (blocks)."""
+        left = np.empty(self.capacity, dtype=np.int64)
+        done = np.empty(self.capacity, dtype=np.int64)
+        self._check(self.lib.mcv_batch_poll(
+            self._b, left.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            done.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return left, done
+
+    def sync(self, slot, fields=STATE_FIELDS, integrals=True):
+        nh = dims(self.info[slot]["dof"], self.info[slot]["n"])[2]
+        rc, out = _read_state(lambda st, m, e: self.lib.mcv_batch_sync(self._b, int(slot), st, m, e),
+                              nh, fields, integrals)
+        self._check(rc)
+        return out
+
+    def autotune(self, reps=3):
+        t = ctypes.c_int32()
+        ms = (ctypes.c_double * TILE_VARIANTS)()
+        self._check(self.lib.mcv_batch_autotune(self._b, int(reps), ctypes.byref(t), ms))
+        return t.value, list(ms)
+
+    def set_tile(self, tile):
+        self._check(self.lib.mcv_batch_set_tile(self._b, int(tile)))
+
+    def set_graphs(self, on):
+        self._check(self.lib.mcv_batch_set_graphs(self._b, 1 if on else 0))
+
+    def run_timed(self, nstages):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.mcv_batch_run_timed(self._b, int(nstages), ctypes.byref(a),
+                                                 ctypes.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self._b:
+            self.lib.mcv_batch_free(self._b)
+            self._b = ctypes.c_void_p()
 
     def __enter__(self):
         return self

@@ -97,6 +97,14 @@ __host__ __device__ inline long long edge_slot(int npv, int p, int side, int fie
   return ((long long)((p - 1) * 4 + side) * 3 + field) * npv + (s - 1);
 }
 
+// (i, j) of point s (1..nPV) along side `side` of a panel.
+__host__ __device__ inline void edge_ij(int npv, int side, int s, int &i, int &j) {
+  if (side == 0) { i = 1; j = s; }
+  else if (side == 1) { i = npv; j = s; }
+  else if (side == 2) { i = s; j = 1; }
+  else { i = s; j = npv; }
+}
+
 // Linear index of point (i, j) of panel p (1-based, halo-relative i, j).
 __host__ __device__ inline long long pidx(int nh, int H, int p, int i, int j) {
   return ((long long)(p - 1) * nh + (j + H - 1)) * (long long)nh + (i + H - 1);
@@ -116,10 +124,33 @@ struct EdgePair { int8_t pa, sa, pb, sb, rev; };
 // A panel corner: panel, i-corner, j-corner (0 -> index 1, 1 -> nPV).
 struct VtxPt { int8_t p, ci, cj; };
 
-extern __constant__ HaloSrc kHalo[6][4];
-extern __constant__ EdgePair kEdges[12];
-extern __constant__ int8_t kFirstEdge[6][4];
-extern __constant__ VtxPt kVerts[8][3];
+// One copy per translation unit (no relocatable device code).
+// Connectivity tables (ghost_mod.f90:104-146, spatial_operators_mod.F90:298-386).
+static __constant__ HaloSrc kHalo[6][4] = {
+    /* P1 */ {{4, 1, 1, 0}, {2, 1, 0, 0}, {6, 0, 1, 0}, {5, 0, 0, 0}},
+    /* P2 */ {{1, 1, 1, 0}, {3, 1, 0, 0}, {6, 1, 1, 1}, {5, 1, 1, 0}},
+    /* P3 */ {{2, 1, 1, 0}, {4, 1, 0, 0}, {6, 0, 0, 1}, {5, 0, 1, 1}},
+    /* P4 */ {{3, 1, 1, 0}, {1, 1, 0, 0}, {6, 1, 0, 0}, {5, 1, 0, 1}},
+    /* P5 */ {{4, 0, 1, 1}, {2, 0, 1, 0}, {1, 0, 1, 0}, {3, 0, 1, 1}},
+    /* P6 */ {{4, 0, 0, 0}, {2, 0, 0, 1}, {3, 0, 0, 1}, {1, 0, 0, 0}},
+};
+static __constant__ EdgePair kEdges[12] = {
+    {1, 0, 4, 1, 0}, {2, 0, 1, 1, 0}, {3, 0, 2, 1, 0}, {4, 0, 3, 1, 0},
+    {1, 3, 5, 2, 0}, {1, 2, 6, 3, 0}, {2, 3, 5, 1, 0}, {2, 2, 6, 1, 1},
+    {3, 3, 5, 3, 1}, {3, 2, 6, 2, 1}, {4, 3, 5, 0, 1}, {4, 2, 6, 0, 0},
+};
+// index into kEdges of the edge whose first panel side is (panel, side), or -1
+static __constant__ int8_t kFirstEdge[6][4] = {
+    {0, -1, 5, 4}, {1, -1, 7, 6}, {2, -1, 9, 8}, {3, -1, 11, 10}, {-1, -1, -1, -1},
+    {-1, -1, -1, -1},
+};
+static __constant__ VtxPt kVerts[8][3] = {
+    {{1, 0, 1}, {4, 1, 1}, {5, 0, 0}}, {{1, 1, 1}, {2, 0, 1}, {5, 1, 0}},
+    {{1, 0, 0}, {4, 1, 0}, {6, 0, 1}}, {{1, 1, 0}, {2, 0, 0}, {6, 1, 1}},
+    {{3, 0, 1}, {2, 1, 1}, {5, 1, 1}}, {{3, 1, 1}, {4, 0, 1}, {5, 0, 1}},
+    {{2, 1, 0}, {3, 0, 0}, {6, 1, 0}}, {{3, 1, 0}, {4, 0, 0}, {6, 0, 0}},
+};
+
 
 // ---------------------------------------------------------------------------
 // MCV formulas (FP64, reference evaluation order)

@@ -18,6 +18,7 @@
 #pragma once
 
 #include "mcv_device.hpp"
+#include "mcv_stage_fp64.hpp"
 
 namespace mcv {
 
@@ -72,24 +73,24 @@ __device__ __forceinline__ float cell_interior_f(const float *f, float dL, float
   }
 }
 
-template <int DOF, int KIND, int CX, int CY>
-__global__ __launch_bounds__(256) void stage_kernel_f32(MeshF32 m, StageArgs32 a, int tiles_x,
-                                                        int ncells) {
+// One (panel p, tile tx, ty) of the FP32C operator + RK stage `kind`; LDS of
+// at least TileLayout<float, DOF, CX, CY>::kBytes.
+template <int DOF, int CX, int CY>
+__device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs32 &a,
+                                               const int kind, const int p, const int tx,
+                                               const int ty, char *smem) {
+  using TL = TileLayout<float, DOF, CX, CY>;
   constexpr int H = DOF - 1;
-  constexpr int NX = CX * H + 1, NY = CY * H + 1;
-  constexpr int TX = NX + 2 * H, TY = NY + 2 * H;
-  __shared__ float sE[TX * TY], sU[TX * TY], sV[TX * TY], sPG[TX * TY], sCU[TX * TY],
-      sCV[TX * TY];
-  __shared__ float lamx[(CX + 1) * NY], lamy[(CY + 1) * NX];
-  __shared__ float rxF[(CX + 1) * NY], rxD[(CX + 1) * NY], rxV[(CX + 1) * NY];
-  __shared__ float ryF[(CY + 1) * NX], ryD[(CY + 1) * NX], ryU[(CY + 1) * NX];
+  constexpr int NX = TL::NX, NY = TL::NY, TX = TL::TX, TY = TL::TY;
+  const TL lds(smem);
+  float *const sE = lds.sE, *const sU = lds.sU, *const sV = lds.sV, *const sPG = lds.sPG,
+              *const sCU = lds.sCU, *const sCV = lds.sCV, *const lamx = lds.lamx,
+              *const lamy = lds.lamy, *const rxF = lds.rxF, *const rxD = lds.rxD,
+              *const rxV = lds.rxV, *const ryF = lds.ryF, *const ryD = lds.ryD,
+              *const ryU = lds.ryU;
 
-  const int npv = m.npv, nh = m.nh, N = ncells;
+  const int npv = m.npv, nh = m.nh, N = m.n;
   const float rdx = m.rdx, r2dx = m.r2dx, rdxx = m.rdxx, dxf = m.dxf;
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int p = L % 6 + 1;
-  const int tile = L / 6;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
   const int cx0 = tx * CX, cy0 = ty * CY;
   const int ncx = min(CX, N - cx0), ncy = min(CY, N - cy0);
   const int i0 = cx0 * H + 1, j0 = cy0 * H + 1;
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(256) void stage_kernel_f32(MeshF32 m, StageArgs32 a
     const double ku = -(m.dEx[g] + (double)fx) + sg * (vort + fc) * cv;
     const double kv = -(m.dEy[g] + (double)fy) - sg * (vort + fc) * cu;
     const double kph = kpg * rsg;  // d(phi')/dt = d(phiG)/dt / sqrtG
-    if constexpr (KIND == kTend) {
+    if (kind == kTend) {
       const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
       a.o_phi[ti] = (float)kpg;
       a.o_u[ti] = (float)ku;
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(256) void stage_kernel_f32(MeshF32 m, StageArgs32 a
     } else {
       // base state Y_n = hi + lo in FP64
       double bph, bu, bv;
-      if constexpr (KIND == kRK4S1 || KIND == kRK3S1) {
+      if (kind == kRK4S1 || kind == kRK3S1) {
         bph = (double)a.in_phi[g] + (double)a.c_phi[g];
         bu = (double)sU[li] + (double)a.c_u[g];
         bv = (double)sV[li] + (double)a.c_v[g];
@@ -269,21 +270,21 @@ __global__ __launch_bounds__(256) void stage_kernel_f32(MeshF32 m, StageArgs32 a
         bv = (double)a.b_v[g] + (double)a.c_v[g];
       }
       double ph, u, v;
-      if constexpr (KIND == kRK4S1) {
+      if (kind == kRK4S1) {
         ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
         a.acc_phiG[g] = (float)kph; a.acc_u[g] = (float)ku; a.acc_v[g] = (float)kv;
-      } else if constexpr (KIND == kRK4S2 || KIND == kRK4S3) {
+      } else if (kind == kRK4S2 || kind == kRK4S3) {
         ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
         a.acc_phiG[g] = (float)((double)a.acc_phiG[g] + 2.0 * kph);
         a.acc_u[g] = (float)((double)a.acc_u[g] + 2.0 * ku);
         a.acc_v[g] = (float)((double)a.acc_v[g] + 2.0 * kv);
-      } else if constexpr (KIND == kRK4S4) {
+      } else if (kind == kRK4S4) {
         ph = bph + a.c1 * (((double)a.acc_phiG[g] + kph) * (1.0 / 6.0));
         u = bu + a.c1 * (((double)a.acc_u[g] + ku) * (1.0 / 6.0));
         v = bv + a.c1 * (((double)a.acc_v[g] + kv) * (1.0 / 6.0));
-      } else if constexpr (KIND == kRK3S1) {
+      } else if (kind == kRK3S1) {
         ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-      } else if constexpr (KIND == kRK3S2) {  // Yn + 1/4 (Y1 - Yn) + 1/4 dt k
+      } else if (kind == kRK3S2) {  // Yn + 1/4 (Y1 - Yn) + 1/4 dt k
         ph = bph + 0.25 * ((double)a.in_phi[g] - bph) + a.c1 * kph;
         u = bu + 0.25 * ((double)sU[li] - bu) + a.c1 * ku;
         v = bv + 0.25 * ((double)sV[li] - bv) + a.c1 * kv;
@@ -327,17 +328,33 @@ __global__ __launch_bounds__(256) void stage_kernel_f32(MeshF32 m, StageArgs32 a
   }
 }
 
+template <int DOF, int KIND, int CX, int CY>
+__global__ __launch_bounds__(kStageThreads) void stage_kernel_f32(MeshF32 m, StageArgs32 a,
+                                                                  int tiles_x) {
+  __shared__ __align__(16) char smem[TileLayout<float, DOF, CX, CY>::kBytes];
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = L / 6;
+  stage_tile_f32<DOF, CX, CY>(m, a, KIND, L % 6 + 1, tile % tiles_x, tile / tiles_x, smem);
+}
+
 // correct_bdy_ghost on an FP32C level: same algorithm as the FP64 boundary
 // kernel, values in FP32 storage, arithmetic and ghost weights in FP64.  The
 // deviation phi' is interpolated and averaged like phi (both operations are
 // linear and phi_ref is already continuous across edges).  For the base state
 // (cphi != NULL) values are hi + lo and every written point gets its residue.
+struct BoundaryArgs32 {
+  float *phi, *u, *v;      // level to finish
+  float *cphi, *cu, *cv;   // its compensation words (base state only, else NULL)
+  const float *edge_raw, *edge_raw_c;
+};
+
 template <int DOF>
-__global__ __launch_bounds__(256) void boundary_kernel_f32(MeshDev m, float *phi, float *u,
-                                                           float *v, float *cphi, float *cu,
-                                                           float *cv, const float *edge_raw,
-                                                           const float *edge_raw_c) {
+__device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const BoundaryArgs32 &ba,
+                                                  const int t) {
   constexpr int H = DOF - 1;
+  float *const phi = ba.phi, *const u = ba.u, *const v = ba.v;
+  float *const cphi = ba.cphi, *const cu = ba.cu, *const cv = ba.cv;
+  const float *const edge_raw = ba.edge_raw, *const edge_raw_c = ba.edge_raw_c;
   const int npv = m.npv, nh = m.nh;
   const double dx = m.dx;
   const int nside = 6 * 4 * npv;
@@ -369,8 +386,7 @@ __global__ __launch_bounds__(256) void boundary_kernel_f32(MeshDev m, float *phi
       cv[g] = (float)(vv - (double)a2);
     }
   };
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nside + 8;
-       t += gridDim.x * blockDim.x) {
+  {
     if (t < nside) {
       const int s = t % npv + 1, ps = t / npv, side = ps % 4, p = ps / 4 + 1;
       const HaloSrc hs = kHalo[p - 1][side];
@@ -444,8 +460,15 @@ __global__ __launch_bounds__(256) void boundary_kernel_f32(MeshDev m, float *phi
   }
 }
 
+template <int DOF>
+__global__ __launch_bounds__(256) void boundary_kernel_f32(MeshDev m, BoundaryArgs32 a) {
+  const int nitems = 6 * 4 * m.npv + 8;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nitems; t += gridDim.x * blockDim.x)
+    boundary_item_f32<DOF>(m, a, t);
+}
+
 // pgref = RN32(phi_ref * sqrtG) on every point.
-__global__ void pgref_kernel(MeshDev m, const double *phi_ref, float *pgref) {
+static __global__ void pgref_kernel(MeshDev m, const double *phi_ref, float *pgref) {
   const long long per = (long long)m.nh * m.nh;
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < 6 * per;
        g += (long long)gridDim.x * blockDim.x) {
@@ -455,7 +478,7 @@ __global__ void pgref_kernel(MeshDev m, const double *phi_ref, float *pgref) {
 }
 
 // FP64 edge-line copy of a finished level (to run the boundary pass on it).
-__global__ void edge_copy_kernel(MeshDev m, const double *phi, const double *u, const double *v,
+static __global__ void edge_copy_kernel(MeshDev m, const double *phi, const double *u, const double *v,
                                  double *edge_raw) {
   const int npv = m.npv, H = m.dof - 1;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 6 * 4 * npv;
@@ -514,7 +537,7 @@ __global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, double *dE
 // Splits the FP64 initial state into the FP32C state: phi' = phi - phi_ref
 // (zero in the interior, the halo-and-edge correction elsewhere), u and v,
 // each rounded to FP32 with its residue as the compensation word.
-__global__ void split_state_kernel(long long np, const double *phi, const double *phi_ref,
+static __global__ void split_state_kernel(long long np, const double *phi, const double *phi_ref,
                                    const double *u, const double *v, float *dphi, float *fu,
                                    float *fv, float *cphi, float *cu, float *cv) {
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
@@ -532,7 +555,7 @@ __global__ void split_state_kernel(long long np, const double *phi, const double
 
 // Reconstructs the FP64 state of an FP32C level: phi = phi_ref + hi + lo,
 // u, v = hi + lo.
-__global__ void join_state_kernel(long long np, const double *phi_ref, const float *dphi,
+static __global__ void join_state_kernel(long long np, const double *phi_ref, const float *dphi,
                                   const float *fu, const float *fv, const float *cphi,
                                   const float *cu, const float *cv, double *phi, double *u,
                                   double *v) {

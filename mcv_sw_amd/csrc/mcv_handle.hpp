@@ -1,0 +1,97 @@
+// Internal: the per-solve handle and helpers shared by the single-solve and
+// batched entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/mcv_hip.h"
+#include "mcv_device.hpp"
+#include "mcv_fp32c.hpp"
+
+namespace mcv {
+
+// Records the message returned by mcv_hip_last_error() and returns code.
+int fail(int code, const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::mcv::fail(MCV_EDEVICE, "%s failed: %s (%s:%d)", #expr,                   \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                    \
+  } while (0)
+
+// Tile shapes (cells x cells per workgroup) the autotuner chooses from, FP64
+// and FP32C.
+constexpr int kTileVariants = 4;
+template <int DOF> struct TileSet;
+template <> struct TileSet<3> {
+  static constexpr int cx[kTileVariants] = {8, 16, 8, 12};
+  static constexpr int cy[kTileVariants] = {8, 4, 4, 6};
+};
+template <> struct TileSet<4> {
+  static constexpr int cx[kTileVariants] = {6, 8, 4, 10};
+  static constexpr int cy[kTileVariants] = {6, 4, 4, 4};
+};
+
+template <int DOF> struct TileSet32;
+template <> struct TileSet32<3> {
+  static constexpr int cx[kTileVariants] = {8, 16, 16, 8};
+  static constexpr int cy[kTileVariants] = {8, 8, 4, 4};
+};
+template <> struct TileSet32<4> {
+  static constexpr int cx[kTileVariants] = {6, 8, 8, 10};
+  static constexpr int cy[kTileVariants] = {6, 4, 8, 6};
+};
+
+
+// Upper bound of the device bytes one solve of this configuration allocates
+// (exact when the metric terms differ between panels).
+size_t footprint_bytes(const mcv_config *c);
+
+// Stage and boundary arguments of a solve's levels (level 0 is Y_n).
+StageArgs stage_args64(const mcv_handle *h, int in, int out, double c1);
+StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1);
+BoundaryArgs boundary_args64(const mcv_handle *h, int level);
+BoundaryArgs32 boundary_args32(const mcv_handle *h, int level);
+
+}  // namespace mcv
+
+// One solve on one device.
+struct mcv_handle {
+  mcv_config cfg{};
+  int H = 0, npv = 0, nh = 0;
+  long long np = 0;  // points per field
+  hipStream_t stream = nullptr;
+  mcv::MeshDev mesh{};
+  std::vector<void *> allocs;  // every device allocation, freed together
+  size_t device_bytes = 0;     // their sizes summed
+  double *lv[3][3] = {};       // level (Yn, S1, S2) x field (phi, u, v)
+  double *acc[3] = {};
+  double *edge_raw = nullptr;
+  double *cu0 = nullptr, *cv0 = nullptr;  // host contra winds until the first stage
+  bool first_pending = true;
+  double *partials = nullptr;
+  int diag_blocks = 0;
+  double *diag_out = nullptr;
+  double *pack[6] = {};
+  int64_t steps = 0;
+  int tile = 0;  // index into the tile shapes of the solve's order
+  // per-kernel timing (mcv_hip_step_timed)
+  bool timing = false;
+  hipEvent_t ev[3] = {};
+  double stage_ms = 0.0, boundary_ms = 0.0;
+  // FP32C form (precision == MCV_FP32C): FP32 levels of (phi - phi_ref, u, v),
+  // compensation words of Y_n, FP32 accumulator and edge copies; phi_ref is
+  // lv[0][0] (the projected initial geopotential, FP64).
+  bool fp32c = false;
+  float *lv32[3][3] = {};
+  float *comp32[3] = {};
+  float *acc32[3] = {};
+  float *edge32 = nullptr, *edge32c = nullptr;
+  double *join[3] = {};  // FP64 reconstruction of Y_n for read-back and integrals
+  mcv::MeshF32 mesh32{};
+};

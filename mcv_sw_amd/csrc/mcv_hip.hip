@@ -3,6 +3,7 @@
 // RK3_TVD and RK4 schemes, read-back and FP64 integrals.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -12,43 +13,12 @@
 #include <vector>
 
 #include "../../include/mcv_hip.h"
-#include "mcv_device.hpp"
+#include "mcv_handle.hpp"
 #include "mcv_kernels_fp64.hpp"
-#include "mcv_fp32c.hpp"
-
-namespace mcv {
-
-// Connectivity tables (ghost_mod.f90:104-146, spatial_operators_mod.F90:298-386).
-__constant__ HaloSrc kHalo[6][4] = {
-    /* P1 */ {{4, 1, 1, 0}, {2, 1, 0, 0}, {6, 0, 1, 0}, {5, 0, 0, 0}},
-    /* P2 */ {{1, 1, 1, 0}, {3, 1, 0, 0}, {6, 1, 1, 1}, {5, 1, 1, 0}},
-    /* P3 */ {{2, 1, 1, 0}, {4, 1, 0, 0}, {6, 0, 0, 1}, {5, 0, 1, 1}},
-    /* P4 */ {{3, 1, 1, 0}, {1, 1, 0, 0}, {6, 1, 0, 0}, {5, 1, 0, 1}},
-    /* P5 */ {{4, 0, 1, 1}, {2, 0, 1, 0}, {1, 0, 1, 0}, {3, 0, 1, 1}},
-    /* P6 */ {{4, 0, 0, 0}, {2, 0, 0, 1}, {3, 0, 0, 1}, {1, 0, 0, 0}},
-};
-__constant__ EdgePair kEdges[12] = {
-    {1, 0, 4, 1, 0}, {2, 0, 1, 1, 0}, {3, 0, 2, 1, 0}, {4, 0, 3, 1, 0},
-    {1, 3, 5, 2, 0}, {1, 2, 6, 3, 0}, {2, 3, 5, 1, 0}, {2, 2, 6, 1, 1},
-    {3, 3, 5, 3, 1}, {3, 2, 6, 2, 1}, {4, 3, 5, 0, 1}, {4, 2, 6, 0, 0},
-};
-// index into kEdges of the edge whose first panel side is (panel, side), or -1
-__constant__ int8_t kFirstEdge[6][4] = {
-    {0, -1, 5, 4}, {1, -1, 7, 6}, {2, -1, 9, 8}, {3, -1, 11, 10}, {-1, -1, -1, -1},
-    {-1, -1, -1, -1},
-};
-__constant__ VtxPt kVerts[8][3] = {
-    {{1, 0, 1}, {4, 1, 1}, {5, 0, 0}}, {{1, 1, 1}, {2, 0, 1}, {5, 1, 0}},
-    {{1, 0, 0}, {4, 1, 0}, {6, 0, 1}}, {{1, 1, 0}, {2, 0, 0}, {6, 1, 1}},
-    {{3, 0, 1}, {2, 1, 1}, {5, 1, 1}}, {{3, 1, 1}, {4, 0, 1}, {5, 0, 1}},
-    {{2, 1, 0}, {3, 0, 0}, {6, 1, 0}}, {{3, 1, 0}, {4, 0, 0}, {6, 0, 0}},
-};
-
-}  // namespace mcv
 
 using namespace mcv;
 
-namespace {
+namespace mcv {
 
 thread_local std::string g_last_error;
 
@@ -62,54 +32,67 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail(MCV_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                  __FILE__, __LINE__);                                                  \
-  } while (0)
+size_t footprint_bytes(const mcv_config *c) {
+  const size_t H = c->dof - 1, npv = (size_t)c->n * H + 1, nh = npv + 2 * H;
+  const size_t np = 6 * nh * nh, nedge = 6 * 4 * 3 * npv, pad = 256;
+  const size_t ncell = 6 * (size_t)c->n * c->n;
+  const size_t diag_blocks = std::min<size_t>((ncell + kDiagThreads - 1) / kDiagThreads, 1024);
+  auto d = [&](size_t count) { return count * sizeof(double) + pad; };
+  auto f = [&](size_t count) { return count * sizeof(float) + pad; };
+  // static mesh data (panel-invariant metric terms counted on every panel)
+  size_t b = 5 * d(np) + d(np) + d(np) + 2 * d(4 * np) + d(H * npv) +
+             sizeof(int32_t) * H * npv + pad;
+  const bool fp32c = c->precision == MCV_FP32C;
+  b += (fp32c ? 3 : 9) * d(np);                       // FP64 levels
+  if (!fp32c) b += 3 * d(np);                          // RK4 accumulator
+  b += d(nedge) + 2 * d(np) + d(4 * diag_blocks) + d(2) + 6 * d(np);
+  if (fp32c)
+    b += 3 * d(np) + 2 * d(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) + 2 * f(nedge) + f(np);
+  return b;
+}
 
-struct DevBuf {
-  double *p = nullptr;
-  size_t n = 0;
-};
+StageArgs stage_args64(const mcv_handle *h, int in, int out, double c1) {
+  StageArgs a{};
+  a.in_phi = h->lv[in][0]; a.in_u = h->lv[in][1]; a.in_v = h->lv[in][2];
+  a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
+  a.acc_phiG = h->acc[0]; a.acc_u = h->acc[1]; a.acc_v = h->acc[2];
+  a.o_phi = h->lv[out][0]; a.o_u = h->lv[out][1]; a.o_v = h->lv[out][2];
+  a.edge_raw = h->edge_raw;
+  a.c1 = c1;
+  return a;
+}
 
-}  // namespace
+BoundaryArgs boundary_args64(const mcv_handle *h, int level) {
+  return BoundaryArgs{h->lv[level][0], h->lv[level][1], h->lv[level][2], h->edge_raw};
+}
 
-struct mcv_handle {
-  mcv_config cfg{};
-  int H = 0, npv = 0, nh = 0;
-  long long np = 0;  // points per field
-  hipStream_t stream = nullptr;
-  MeshDev mesh{};
-  std::vector<void *> allocs;  // every device allocation, freed together
-  double *lv[3][3] = {};       // level (Yn, S1, S2) x field (phi, u, v)
-  double *acc[3] = {};
-  double *edge_raw = nullptr;
-  double *cu0 = nullptr, *cv0 = nullptr;  // host contra winds until the first stage
-  bool first_pending = true;
-  double *partials = nullptr;
-  int diag_blocks = 0;
-  double *diag_out = nullptr;
-  double *pack[6] = {};
-  int64_t steps = 0;
-  int tile = 0;  // index into the tile shapes of the solve's order
-  // per-kernel timing (mcv_hip_step_timed)
-  bool timing = false;
-  hipEvent_t ev[3] = {};
-  double stage_ms = 0.0, boundary_ms = 0.0;
-  // FP32C form (precision == MCV_FP32C): FP32 levels of (phi - phi_ref, u, v),
-  // compensation words of Y_n, FP32 accumulator and edge copies; phi_ref is
-  // lv[0][0] (the projected initial geopotential, FP64).
-  bool fp32c = false;
-  float *lv32[3][3] = {};
-  float *comp32[3] = {};
-  float *acc32[3] = {};
-  float *edge32 = nullptr, *edge32c = nullptr;
-  double *join[3] = {};  // FP64 reconstruction of Y_n for read-back and integrals
-  MeshF32 mesh32{};
-};
+StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1) {
+  StageArgs32 a{};
+  a.in_phi = h->lv32[in][0]; a.in_u = h->lv32[in][1]; a.in_v = h->lv32[in][2];
+  a.b_phi = h->lv32[0][0]; a.b_u = h->lv32[0][1]; a.b_v = h->lv32[0][2];
+  a.c_phi = h->comp32[0]; a.c_u = h->comp32[1]; a.c_v = h->comp32[2];
+  a.acc_phiG = h->acc32[0]; a.acc_u = h->acc32[1]; a.acc_v = h->acc32[2];
+  a.o_phi = h->lv32[out][0]; a.o_u = h->lv32[out][1]; a.o_v = h->lv32[out][2];
+  if (out == 0) {  // the step's last stage writes Y_n+1 with its compensation
+    a.oc_phi = h->comp32[0]; a.oc_u = h->comp32[1]; a.oc_v = h->comp32[2];
+  }
+  a.edge_raw = h->edge32;
+  a.edge_raw_c = h->edge32c;
+  a.c1 = c1;
+  return a;
+}
+
+BoundaryArgs32 boundary_args32(const mcv_handle *h, int level) {
+  const bool base = level == 0;  // Y_n carries its compensation words
+  BoundaryArgs32 b{};
+  b.phi = h->lv32[level][0]; b.u = h->lv32[level][1]; b.v = h->lv32[level][2];
+  if (base) { b.cphi = h->comp32[0]; b.cu = h->comp32[1]; b.cv = h->comp32[2]; }
+  b.edge_raw = h->edge32;
+  b.edge_raw_c = h->edge32c;
+  return b;
+}
+
+}  // namespace mcv
 
 namespace {
 
@@ -119,6 +102,7 @@ int dev_alloc(mcv_handle *h, double **out, size_t count) {
   if (e != hipSuccess)
     return fail(MCV_ENOMEM, "hipMalloc(%zu doubles) failed: %s", count, hipGetErrorString(e));
   h->allocs.push_back(p);
+  h->device_bytes += count * sizeof(double) + 256;
   *out = static_cast<double *>(p);
   return MCV_OK;
 }
@@ -167,18 +151,6 @@ int validate(const mcv_config *c) {
   return MCV_OK;
 }
 
-// Tile shapes (cells x cells per workgroup) the autotuner chooses from.
-constexpr int kTileVariants = 4;
-template <int DOF> struct TileSet;
-template <> struct TileSet<3> {
-  static constexpr int cx[kTileVariants] = {8, 16, 8, 12};
-  static constexpr int cy[kTileVariants] = {8, 4, 4, 6};
-};
-template <> struct TileSet<4> {
-  static constexpr int cx[kTileVariants] = {6, 8, 4, 10};
-  static constexpr int cy[kTileVariants] = {6, 4, 4, 4};
-};
-
 template <int DOF, int CX, int CY>
 int launch_stage_shape(mcv_handle *h, int kind, const StageArgs &a) {
   const int tx = (h->cfg.n + CX - 1) / CX, ty = (h->cfg.n + CY - 1) / CY;
@@ -221,7 +193,7 @@ int stage(mcv_handle *h, int kind, const StageArgs &a) {
 }
 
 int boundary(mcv_handle *h, int level) {
-  BoundaryArgs b{h->lv[level][0], h->lv[level][1], h->lv[level][2], h->edge_raw};
+  const BoundaryArgs b = boundary_args64(h, level);
   const int threads = 6 * 4 * h->npv + 8;
   dim3 grid((threads + 255) / 256), block(256);
   if (h->cfg.dof == 3)
@@ -235,16 +207,6 @@ int boundary(mcv_handle *h, int level) {
 // ---------------------------------------------------------------------------
 // FP32C launches
 // ---------------------------------------------------------------------------
-template <int DOF> struct TileSet32;
-template <> struct TileSet32<3> {
-  static constexpr int cx[kTileVariants] = {8, 16, 16, 8};
-  static constexpr int cy[kTileVariants] = {8, 8, 4, 4};
-};
-template <> struct TileSet32<4> {
-  static constexpr int cx[kTileVariants] = {6, 8, 8, 10};
-  static constexpr int cy[kTileVariants] = {6, 4, 8, 6};
-};
-
 template <int DOF, int CX, int CY>
 int launch_stage32_shape(mcv_handle *h, int kind, const StageArgs32 &a) {
   const int tx = (h->cfg.n + CX - 1) / CX, ty = (h->cfg.n + CY - 1) / CY;
@@ -253,7 +215,7 @@ int launch_stage32_shape(mcv_handle *h, int kind, const StageArgs32 &a) {
 #define MCV_CASE(K)                                                                       \
   case K:                                                                                 \
     hipLaunchKernelGGL((stage_kernel_f32<DOF, K, CX, CY>), grid, block, 0, h->stream,      \
-                       h->mesh32, a, tx, h->cfg.n);                                       \
+                       h->mesh32, a, tx);                                                 \
     break;
     MCV_CASE(kRK4S1)
     MCV_CASE(kRK4S2)
@@ -285,56 +247,29 @@ int stage32(mcv_handle *h, int kind, const StageArgs32 &a) {
   return h->cfg.dof == 3 ? launch_stage32<3>(h, kind, a) : launch_stage32<4>(h, kind, a);
 }
 
-int boundary32(mcv_handle *h, int level, bool base) {
-  float *c0 = base ? h->comp32[0] : nullptr, *c1 = base ? h->comp32[1] : nullptr,
-        *c2 = base ? h->comp32[2] : nullptr;
+int boundary32(mcv_handle *h, int level) {
+  const BoundaryArgs32 b = boundary_args32(h, level);
   const int threads = 6 * 4 * h->npv + 8;
   dim3 grid((threads + 255) / 256), block(256);
   if (h->cfg.dof == 3)
-    hipLaunchKernelGGL(boundary_kernel_f32<3>, grid, block, 0, h->stream, h->mesh,
-                       h->lv32[level][0], h->lv32[level][1], h->lv32[level][2], c0, c1, c2,
-                       h->edge32, h->edge32c);
+    hipLaunchKernelGGL(boundary_kernel_f32<3>, grid, block, 0, h->stream, h->mesh, b);
   else
-    hipLaunchKernelGGL(boundary_kernel_f32<4>, grid, block, 0, h->stream, h->mesh,
-                       h->lv32[level][0], h->lv32[level][1], h->lv32[level][2], c0, c1, c2,
-                       h->edge32, h->edge32c);
+    hipLaunchKernelGGL(boundary_kernel_f32<4>, grid, block, 0, h->stream, h->mesh, b);
   HIP_TRY(hipGetLastError());
   return MCV_OK;
-}
-
-StageArgs32 stage_args32(mcv_handle *h, int in, int out, double c1) {
-  StageArgs32 a{};
-  a.in_phi = h->lv32[in][0]; a.in_u = h->lv32[in][1]; a.in_v = h->lv32[in][2];
-  a.b_phi = h->lv32[0][0]; a.b_u = h->lv32[0][1]; a.b_v = h->lv32[0][2];
-  a.c_phi = h->comp32[0]; a.c_u = h->comp32[1]; a.c_v = h->comp32[2];
-  a.acc_phiG = h->acc32[0]; a.acc_u = h->acc32[1]; a.acc_v = h->acc32[2];
-  a.o_phi = h->lv32[out][0]; a.o_u = h->lv32[out][1]; a.o_v = h->lv32[out][2];
-  if (out == 0) {  // the step's last stage writes Y_n+1 with its compensation
-    a.oc_phi = h->comp32[0]; a.oc_u = h->comp32[1]; a.oc_v = h->comp32[2];
-  }
-  a.edge_raw = h->edge32;
-  a.edge_raw_c = h->edge32c;
-  a.c1 = c1;
-  return a;
 }
 
 // One RK stage: stage kernel from level `in` (base Yn = level 0) into level
 // `out`, then the boundary pass on `out`.
 int rk_stage(mcv_handle *h, int kind, int in, int out, double c1) {
-  StageArgs a{};
-  a.in_phi = h->lv[in][0]; a.in_u = h->lv[in][1]; a.in_v = h->lv[in][2];
-  a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
-  a.acc_phiG = h->acc[0]; a.acc_u = h->acc[1]; a.acc_v = h->acc[2];
-  a.o_phi = h->lv[out][0]; a.o_u = h->lv[out][1]; a.o_v = h->lv[out][2];
-  a.edge_raw = h->edge_raw;
+  StageArgs a = h->fp32c ? StageArgs{} : stage_args64(h, in, out, c1);
   if (h->first_pending) { a.cu0 = h->cu0; a.cv0 = h->cv0; }
-  a.c1 = c1;
   if (h->timing) HIP_TRY(hipEventRecord(h->ev[0], h->stream));
   int rc = h->fp32c ? stage32(h, kind, stage_args32(h, in, out, c1)) : stage(h, kind, a);
   if (rc) return rc;
   h->first_pending = false;
   if (h->timing) HIP_TRY(hipEventRecord(h->ev[1], h->stream));
-  rc = h->fp32c ? boundary32(h, out, out == 0) : boundary(h, out);
+  rc = h->fp32c ? boundary32(h, out) : boundary(h, out);
   if (rc) return rc;
   if (h->timing) {
     // events bracket each kernel on the handle's stream; wait so they can be reused
@@ -417,6 +352,7 @@ int dev_alloc_t(mcv_handle *h, T **out, size_t count) {
     return fail(MCV_ENOMEM, "hipMalloc(%zu bytes) failed: %s", count * sizeof(T),
                 hipGetErrorString(e));
   h->allocs.push_back(p);
+  h->device_bytes += count * sizeof(T) + 256;
   *out = static_cast<T *>(p);
   return MCV_OK;
 }
@@ -600,8 +536,9 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   m.gcoef = d;
   {
     void *pi = nullptr;
-    HIP_TRY(hipMalloc(&pi, sizeof(int32_t) * H * h->npv));
+    HIP_TRY(hipMalloc(&pi, sizeof(int32_t) * H * h->npv + 256));
     h->allocs.push_back(pi);
+    h->device_bytes += sizeof(int32_t) * H * h->npv + 256;
     HIP_TRY(hipMemcpy(pi, mesh->ghost_iref, sizeof(int32_t) * H * h->npv,
                       hipMemcpyHostToDevice));
     m.giref = static_cast<const int32_t *>(pi);
@@ -750,6 +687,7 @@ int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG, double 
   for (int f = 0; f < 3; ++f) {
     (void)hipFree(tk[f]);
     h->allocs.pop_back();
+    h->device_bytes -= (size_t)6 * h->npv * h->npv * sizeof(double) + 256;
   }
   return MCV_OK;
 }
@@ -830,6 +768,20 @@ int mcv_hip_steps_done(const mcv_handle *h, int64_t *steps) {
 
 int mcv_hip_free(mcv_handle *h) {
   destroy(h);
+  return MCV_OK;
+}
+
+int mcv_hip_footprint(const mcv_config *cfg, int64_t *bytes) {
+  if (!bytes) return fail(MCV_EINVAL, "bytes is NULL");
+  int rc = validate(cfg);
+  if (rc) return rc;
+  *bytes = (int64_t)footprint_bytes(cfg);
+  return MCV_OK;
+}
+
+int mcv_hip_device_bytes(const mcv_handle *h, int64_t *bytes) {
+  if (!h || !bytes) return fail(MCV_EINVAL, "NULL argument");
+  *bytes = (int64_t)h->device_bytes;
   return MCV_OK;
 }
 

@@ -41,23 +41,16 @@ __device__ __forceinline__ void raw_point(const MeshDev &m, const BoundaryArgs &
   }
 }
 
-__device__ __forceinline__ void edge_ij(int npv, int side, int s, int &i, int &j) {
-  if (side == 0) { i = 1; j = s; }
-  else if (side == 1) { i = npv; j = s; }
-  else if (side == 2) { i = s; j = 1; }
-  else { i = s; j = npv; }
-}
-
 // correct_bdy_ghost (time_scheme_mod.F90:125-142) on one level: one thread per
 // (panel, side, s), plus 8 threads for the cube vertices.
+// Work item t of correct_bdy_ghost on one level, t in [0, 6*4*nPV + 8).
 template <int DOF>
-__global__ __launch_bounds__(256) void boundary_kernel(MeshDev m, BoundaryArgs a) {
+__device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryArgs &a,
+                                              const int t) {
   constexpr int H = DOF - 1;
   const int npv = m.npv, nh = m.nh;
   const double dx = m.dx;
   const int nside = 6 * 4 * npv;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nside + 8;
-       t += gridDim.x * blockDim.x) {
   if (t < nside) {
     const int s = t % npv + 1, ps = t / npv, side = ps % 4, p = ps / 4 + 1;
     const HaloSrc hs = kHalo[p - 1][side];
@@ -145,12 +138,18 @@ __global__ __launch_bounds__(256) void boundary_kernel(MeshDev m, BoundaryArgs a
       a.v[gg[l]] = v;
     }
   }
-  }
+}
+
+template <int DOF>
+__global__ __launch_bounds__(256) void boundary_kernel(MeshDev m, BoundaryArgs a) {
+  const int nitems = 6 * 4 * m.npv + 8;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nitems; t += gridDim.x * blockDim.x)
+    boundary_item<DOF>(m, a, t);
 }
 
 // Reference-layout copy-out of one level: phi, phiG = phi*sqrtG, u, v and
 // the contravariant winds (or the host's initial ones before the first step).
-__global__ void pack_kernel(MeshDev m, const double *phi, const double *u, const double *v,
+static __global__ void pack_kernel(MeshDev m, const double *phi, const double *u, const double *v,
                             const double *cu0, const double *cv0, double *o_phi,
                             double *o_phiG, double *o_u, double *o_v, double *o_cu,
                             double *o_cv) {
@@ -176,7 +175,7 @@ __global__ void pack_kernel(MeshDev m, const double *phi, const double *u, const
 }
 
 // Sphere winds on every point (spatial_operators_mod.F90:391-405).
-__global__ void sphere_winds_kernel(MeshDev m, const double *u, const double *v, double *zon,
+static __global__ void sphere_winds_kernel(MeshDev m, const double *u, const double *v, double *zon,
                                     double *mer) {
   const long long np = 6LL * m.nh * m.nh;
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
@@ -266,7 +265,7 @@ __global__ __launch_bounds__(kDiagThreads) void diag_partial_kernel(MeshDev m, c
   }
 }
 
-__global__ void diag_final_kernel(const double *partials, int nblocks, double *out) {
+static __global__ void diag_final_kernel(const double *partials, int nblocks, double *out) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     DD ms{0.0, 0.0}, es{0.0, 0.0};
     for (int b = 0; b < nblocks; ++b) {

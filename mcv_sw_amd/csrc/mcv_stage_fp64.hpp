@@ -17,30 +17,50 @@ namespace mcv {
 
 constexpr int kStageThreads = 256;
 
-// Tile of CX x CY cells per workgroup (the autotuner picks among the shapes
-// instantiated in mcv_hip.hip).
-template <int DOF, int KIND, int CX, int CY>
-__global__ __launch_bounds__(kStageThreads) void stage_kernel(MeshDev m, StageArgs a,
-                                                              int tiles_x) {
+// LDS layout of one FP64 tile of CX x CY cells, carved from a byte buffer so
+// that one kernel can host tiles of several orders and precisions.
+template <typename T, int DOF, int CX, int CY>
+struct TileLayout {
+  static constexpr int H = DOF - 1;
+  static constexpr int NX = CX * H + 1;  // core extents incl. the far cell end
+  static constexpr int NY = CY * H + 1;
+  static constexpr int TX = NX + 2 * H;  // staged extents incl. the halo cell
+  static constexpr int TY = NY + 2 * H;
+  static constexpr int kPts = TX * TY;
+  static constexpr int kEx = (CX + 1) * NY;  // x-sweep cell ends: (CX+1) per core row
+  static constexpr int kEy = (CY + 1) * NX;  // y-sweep: (CY+1) per core column
+  static constexpr size_t kBytes = sizeof(T) * (6 * kPts + 4 * kEx + 4 * kEy);
+  T *sE, *sU, *sV, *sPG, *sCU, *sCV, *lamx, *lamy, *rxF, *rxD, *rxV, *ryF, *ryD, *ryU;
+  __device__ __forceinline__ explicit TileLayout(char *smem) {
+    T *q = reinterpret_cast<T *>(smem);
+    sE = q; q += kPts; sU = q; q += kPts; sV = q; q += kPts;
+    sPG = q; q += kPts; sCU = q; q += kPts; sCV = q; q += kPts;
+    lamx = q; q += kEx; lamy = q; q += kEy;
+    rxF = q; q += kEx; rxD = q; q += kEx; rxV = q; q += kEx;
+    ryF = q; q += kEy; ryD = q; q += kEy; ryU = q;
+  }
+};
+
+// One (panel p, tile tx, ty) of the spatial operator + RK stage `kind`.  The
+// caller passes LDS of at least TileLayout<double, DOF, CX, CY>::kBytes and
+// synchronises before reusing it.
+template <int DOF, int CX, int CY>
+__device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a, const int kind,
+                                           const int p, const int tx, const int ty, char *smem) {
+  using TL = TileLayout<double, DOF, CX, CY>;
   constexpr int H = DOF - 1;
-  constexpr int NX = CX * H + 1;   // core extents incl. the far cell end
-  constexpr int NY = CY * H + 1;
-  constexpr int TX = NX + 2 * H;   // staged extents incl. the halo cell
-  constexpr int TY = NY + 2 * H;
-  __shared__ double sE[TX * TY], sU[TX * TY], sV[TX * TY], sPG[TX * TY], sCU[TX * TY],
-      sCV[TX * TY];
-  // x-sweep cell ends: (CX+1) per core row; y-sweep: (CY+1) per core column
-  __shared__ double lamx[(CX + 1) * NY], lamy[(CY + 1) * NX];
-  __shared__ double rxF[(CX + 1) * NY], rxD[(CX + 1) * NY], rxV[(CX + 1) * NY];
-  __shared__ double ryF[(CY + 1) * NX], ryD[(CY + 1) * NX], ryU[(CY + 1) * NX];
+  constexpr int NX = TL::NX, NY = TL::NY, TX = TL::TX;
+  constexpr int TY = TL::TY;
+  const TL lds(smem);
+  double *const sE = lds.sE, *const sU = lds.sU, *const sV = lds.sV, *const sPG = lds.sPG,
+               *const sCU = lds.sCU, *const sCV = lds.sCV, *const lamx = lds.lamx,
+               *const lamy = lds.lamy, *const rxF = lds.rxF, *const rxD = lds.rxD,
+               *const rxV = lds.rxV, *const ryF = lds.ryF, *const ryD = lds.ryD,
+               *const ryU = lds.ryU;
 
   const int npv = m.npv, nh = m.nh, N = m.n;
   const double dx = m.dx;
   const MCVDivisors dv = m.dv;
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int p = L % 6 + 1;
-  const int tile = L / 6;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
   const int cx0 = tx * CX, cy0 = ty * CY;
   const int ncx = min(CX, N - cx0), ncy = min(CY, N - cy0);
   const int i0 = cx0 * H + 1, j0 = cy0 * H + 1;
@@ -206,36 +226,36 @@ __global__ __launch_bounds__(kStageThreads) void stage_kernel(MeshDev m, StageAr
     const double ku = -fx + sg * (vort + fc) * cv;
     const double kv = -fy - sg * (vort + fc) * cu;
 
-    if constexpr (KIND == kTend) {
+    if (kind == kTend) {
       const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
       a.o_phi[ti] = kpg;
       a.o_u[ti] = ku;
       a.o_v[ti] = kv;
     } else {
       double pgn, un, vn;  // base state Y_n
-      if constexpr (KIND == kRK4S1 || KIND == kRK3S1) {
+      if (kind == kRK4S1 || kind == kRK3S1) {
         pgn = sPG[li]; un = sU[li]; vn = sV[li];
       } else {
         pgn = a.b_phi[g] * sg; un = a.b_u[g]; vn = a.b_v[g];
       }
       double pg, u, v;
-      if constexpr (KIND == kRK4S1) {
+      if (kind == kRK4S1) {
         pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
         a.acc_phiG[g] = kpg; a.acc_u[g] = ku; a.acc_v[g] = kv;
-      } else if constexpr (KIND == kRK4S2 || KIND == kRK4S3) {
+      } else if (kind == kRK4S2 || kind == kRK4S3) {
         pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
         a.acc_phiG[g] = a.acc_phiG[g] + 2.0 * kpg;
         a.acc_u[g] = a.acc_u[g] + 2.0 * ku;
         a.acc_v[g] = a.acc_v[g] + 2.0 * kv;
-      } else if constexpr (KIND == kRK4S4) {
+      } else if (kind == kRK4S4) {
         constexpr Recip k6{6.0, 1.0 / 6.0};
         const double tpg = div_by(a.acc_phiG[g] + kpg, k6);
         const double tu = div_by(a.acc_u[g] + ku, k6);
         const double tv = div_by(a.acc_v[g] + kv, k6);
         pg = pgn + a.c1 * tpg; u = un + a.c1 * tu; v = vn + a.c1 * tv;
-      } else if constexpr (KIND == kRK3S1) {
+      } else if (kind == kRK3S1) {
         pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-      } else if constexpr (KIND == kRK3S2) {
+      } else if (kind == kRK3S2) {
         pg = 0.75 * pgn + 0.25 * sPG[li] + a.c1 * kpg;
         u = 0.75 * un + 0.25 * sU[li] + a.c1 * ku;
         v = 0.75 * vn + 0.25 * sV[li] + a.c1 * kv;
@@ -262,6 +282,17 @@ __global__ __launch_bounds__(kStageThreads) void stage_kernel(MeshDev m, StageAr
       }
     }
   }
+}
+
+// Single-solve stage kernel: one workgroup per (panel, tile), blocks of one
+// XCD on neighbouring tiles.
+template <int DOF, int KIND, int CX, int CY>
+__global__ __launch_bounds__(kStageThreads) void stage_kernel(MeshDev m, StageArgs a,
+                                                              int tiles_x) {
+  __shared__ __align__(16) char smem[TileLayout<double, DOF, CX, CY>::kBytes];
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = L / 6;
+  stage_tile<DOF, CX, CY>(m, a, KIND, L % 6 + 1, tile % tiles_x, tile / tiles_x, smem);
 }
 
 }  // namespace mcv

@@ -61,6 +61,72 @@ static int run(int dof, int n, int scheme, int precision) {
   return 0;
 }
 
+// Batch of mixed solves: graphs, autotune, admission mid-step, slot reuse.
+static int run_batch() {
+  struct Spec { int dof, n, scheme, precision; int64_t steps; };
+  const Spec specs[] = {{3, 5, MCV_RK4, MCV_FP64, 2}, {4, 3, MCV_RK3_TVD, MCV_FP32C, 3},
+                        {4, 7, MCV_RK4, MCV_FP32C, 1}, {3, 2, MCV_RK3_TVD, MCV_FP64, 2}};
+  mcv_batch *b = nullptr;
+  if (mcv_batch_create(0, 3, &b)) { std::printf("create: %s\n", mcv_hip_last_error()); return 1; }
+  std::vector<std::vector<double>> keep;
+  int32_t slots[4];
+  for (int k = 0; k < 4; ++k) {
+    const Spec &sp = specs[k];
+    const int H = sp.dof - 1, npv = sp.n * H + 1, nh = npv + 2 * H;
+    const size_t np = (size_t)6 * nh * nh;
+    std::vector<double> sg(np), G(4 * np), IG(4 * np), A(4 * np), IA(4 * np), f(np), ps(np);
+    std::vector<double> phi(np), phiG(np), u(np), v(np), cu(np), cv(np), coef(H * npv);
+    std::vector<int32_t> iref(H * npv);
+    for (size_t q = 0; q < np; ++q) {
+      double x = 1e-3 * (double)(q % 977);
+      sg[q] = 1.0 + 0.1 * std::sin(x);
+      G[4 * q] = 1.0; G[4 * q + 1] = 0.0; G[4 * q + 2] = 0.0; G[4 * q + 3] = 1.0;
+      IG[4 * q] = 1.0; IG[4 * q + 1] = 0.0; IG[4 * q + 2] = 0.0; IG[4 * q + 3] = 1.0;
+      A[4 * q] = 1.0; A[4 * q + 1] = 0.0; A[4 * q + 2] = 0.0; A[4 * q + 3] = 1.0;
+      IA[4 * q] = 1.0; IA[4 * q + 1] = 0.0; IA[4 * q + 2] = 0.0; IA[4 * q + 3] = 1.0;
+      f[q] = 1e-4; ps[q] = 0.0;
+      phi[q] = 5e4 + 10.0 * std::cos(3 * x); phiG[q] = phi[q] * sg[q];
+      u[q] = 1.0; v[q] = 0.5; cu[q] = u[q]; cv[q] = v[q];
+    }
+    const double dx = (M_PI / 2) / sp.n;
+    for (int q = 0; q < H * npv; ++q) { iref[q] = 1 + (q / H) % sp.n; coef[q] = 0.3 * dx; }
+    mcv_config cfg{MCV_ABI_VERSION, sp.dof, sp.n, sp.scheme, sp.precision, 0, 1e-6, dx};
+    mcv_mesh mesh{sg.data(), G.data(), IG.data(), A.data(), IA.data(), f.data(), ps.data(),
+                  coef.data(), iref.data()};
+    mcv_state st{phi.data(), phiG.data(), u.data(), v.data(), cu.data(), cv.data()};
+    if (k == 3) {  // the batch is full until slot 0 is retired
+      if (mcv_batch_add(b, &cfg, &mesh, &st, sp.steps, &slots[k]) != MCV_ESTATE) return 1;
+      int64_t left[3];
+      for (int it = 0; it < 50; ++it) {
+        if (mcv_batch_poll(b, left, nullptr)) return 1;
+        if (left[slots[0]] == 0) break;
+        if (mcv_batch_run(b, 1)) return 1;
+      }
+      if (mcv_batch_remove(b, slots[0])) return 1;
+    }
+    if (mcv_batch_add(b, &cfg, &mesh, &st, sp.steps, &slots[k])) { std::printf("add: %s\n", mcv_hip_last_error()); return 1; }
+    if (k == 0) {
+      int32_t t;
+      if (mcv_batch_autotune(b, 1, &t, nullptr)) { std::printf("autotune: %s\n", mcv_hip_last_error()); return 1; }
+    }
+    if (mcv_batch_run(b, k == 1 ? 13 : 2)) { std::printf("run: %s\n", mcv_hip_last_error()); return 1; }
+  }
+  if (mcv_batch_run(b, 30)) return 1;
+  int64_t left[3], done[3];
+  if (mcv_batch_poll(b, left, done)) return 1;
+  for (int k = 1; k < 4; ++k) {
+    const int H = specs[k].dof - 1, npv = specs[k].n * H + 1, nh = npv + 2 * H;
+    std::vector<double> phi((size_t)6 * nh * nh);
+    mcv_state o{phi.data(), nullptr, nullptr, nullptr, nullptr, nullptr};
+    double mass;
+    if (left[slots[k]] != 0 || done[slots[k]] != specs[k].steps) { std::printf("batch progress\n"); return 1; }
+    if (mcv_batch_sync(b, slots[k], &o, &mass, nullptr)) { std::printf("bsync: %s\n", mcv_hip_last_error()); return 1; }
+  }
+  mcv_batch_free(b);
+  std::printf("batch ok\n");
+  return 0;
+}
+
 int main() {
   setvbuf(stdout, nullptr, _IONBF, 0);
   int rc = 0;
@@ -68,5 +134,6 @@ int main() {
     for (int n : {2, 5, 7, 13})
       for (int scheme : {MCV_RK4, MCV_RK3_TVD})
         for (int precision : {MCV_FP64, MCV_FP32C}) rc |= run(dof, n, scheme, precision);
+  rc |= run_batch();
   return rc;
 }

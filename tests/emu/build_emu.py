@@ -13,7 +13,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
-SRCS = [os.path.join(ROOT, "mcv_sw_amd", "csrc", f) for f in ("mcv_hip.hip", "mcv_probe.hip")]
+SRCS = [os.path.join(ROOT, "mcv_sw_amd", "csrc", f) for f in ("mcv_hip.hip", "mcv_batch.hip", "mcv_probe.hip")]
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libmcv_hip_emu.so")
 ASAN = os.path.join(OUT, "asan_driver")

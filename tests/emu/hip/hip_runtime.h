@@ -16,6 +16,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <vector>
 
 #define __global__
 #define __device__
@@ -24,6 +26,8 @@
 #define __shared__ static
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define __align__(n) alignas(n)
+#define __restrict__
 
 struct dim3 {
   unsigned x, y, z;
@@ -31,6 +35,12 @@ struct dim3 {
 };
 inline thread_local dim3 blockIdx(0), threadIdx(0), blockDim(1), gridDim(1);
 inline void __syncthreads() {}
+inline void __threadfence() {}
+inline unsigned atomicAdd(unsigned *p, unsigned v) {
+  const unsigned o = *p;
+  *p = o + v;
+  return o;
+}
 using std::max;
 using std::min;
 
@@ -85,13 +95,67 @@ inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
   return hipSuccess;
 }
 
+// Device properties of the emulated device.
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount };
+inline hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) {
+  *v = 4;
+  return hipSuccess;
+}
+template <typename K>
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, K, int, size_t) {
+  *n = 2;
+  return hipSuccess;
+}
+
+// Stream capture records launches as closures; a graph launch replays them.
+struct EmuGraph {
+  std::vector<std::function<void()>> ops;
+};
+typedef EmuGraph *hipGraph_t;
+typedef EmuGraph *hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal };
+inline thread_local EmuGraph *g_emu_capture = nullptr;
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  g_emu_capture = new EmuGraph();
+  return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+  *g = g_emu_capture;
+  g_emu_capture = nullptr;
+  return hipSuccess;
+}
+inline hipError_t hipGraphInstantiateWithFlags(hipGraphExec_t *e, hipGraph_t g,
+                                               unsigned long long) {
+  *e = new EmuGraph(*g);
+  return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto &f : e->ops) f();
+  return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+  delete e;
+  return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+  delete g;
+  return hipSuccess;
+}
+
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)          \
   do {                                                                       \
-    gridDim = dim3(grid);                                                    \
-    blockDim = dim3(1);                                                      \
-    threadIdx = dim3(0);                                                     \
-    for (unsigned bx_ = 0; bx_ < gridDim.x; ++bx_) {                         \
-      blockIdx = dim3(bx_);                                                  \
-      kernel(__VA_ARGS__);                                                   \
-    }                                                                        \
+    const dim3 g_ = dim3(grid);                                              \
+    auto op_ = [=]() {                                                       \
+      gridDim = g_;                                                          \
+      blockDim = dim3(1);                                                    \
+      threadIdx = dim3(0);                                                   \
+      for (unsigned bx_ = 0; bx_ < gridDim.x; ++bx_) {                       \
+        blockIdx = dim3(bx_);                                                \
+        kernel(__VA_ARGS__);                                                 \
+      }                                                                      \
+    };                                                                       \
+    if (g_emu_capture)                                                       \
+      g_emu_capture->ops.push_back(op_);                                     \
+    else                                                                     \
+      op_();                                                                 \
   } while (0)
