@@ -61,7 +61,10 @@ _libs = {}
 
 
 def load(path=None):
-    """Load the C-ABI library (default: the in-tree HIP build)."""
+    """Load the C-ABI library (default: the in-tree HIP build); an already
+    loaded library is returned as is."""
+    if isinstance(path, ctypes.CDLL):
+        return path
     path = os.path.abspath(path or LIB_PATH)
     if path in _libs:
         return _libs[path]
