@@ -13,13 +13,17 @@ Also reported on the same line:
                cell-steps/s, and its h/u/v relative L2 and Linf distance and
                mass drift from the FP64 trajectory after the same steps
   batched      the uniform ensemble: 64 members of MCV3 on 6 x 90 x 90, RK3,
-               dt = 300 s, FP32C, member k perturbed with seed k (degree 12,
+               dt = 60 s, FP32C, member k perturbed with seed k (degree 12,
                20 m rms) on the balanced jet u0 = 20 m/s, h0 = 5960 m; one
                fused stage launch + one fused boundary launch per RK stage for
                all members, replayed from a hipGraph
   roofline     the FP64 stage kernel's achieved algorithmic HBM bandwidth
                (51 FP64 words per point per RK4 step, 9 points per MCV4 cell)
                from HIP events on the solver's stream, against 8 TB/s
+  serve        the serving loop: Poisson arrivals (20 req/s) of seeded solves
+               of mixed order, size, length and precision, continuous batching
+               with admission by HBM footprint; completed solves/s and p50/p99
+               latency
   cpu_baseline the reference Fortran (oracle/_ref, built from the reference's
                own sources) on this host's cores, on a 2-step sample of the
                headline workload
@@ -44,7 +48,8 @@ DT = 10.0  # s; the configured 60 s is unstable for the reference scheme (Couran
 U0, H0, LMAX, SEED, RMS = 40.0, 8000.0, 16, 0, 60.0
 GRAVITY = 9.80616
 # uniform ensemble
-E_MEMBERS, E_DOF, E_N, E_DT, E_SCHEME = 64, 3, 90, 300.0, "RK3_TVD"
+# dt = 60 s: the configured 300 s (point Courant ~1.4) blows up within a day
+E_MEMBERS, E_DOF, E_N, E_DT, E_SCHEME = 64, 3, 90, 60.0, "RK3_TVD"
 E_U0, E_H0, E_LMAX, E_RMS = 20.0, 5960.0, 12, 20.0
 
 
@@ -59,15 +64,15 @@ def words_per_point(scheme):
 
 def fp32c_bytes_per_point_step(scheme):
     """Algorithmic HBM bytes per unique point per time step of the FP32C stage
-    kernels: FP32 stage input, base hi and lo, accumulator, output and (last
-    stage) output residue; FP32 phi_ref*sqrtG, FP64 reference gradient (2
-    words) and FP64 Coriolis parameter per stage."""
+    kernels: FP32 stage input, base hi (stages 2..), base lo (last stage),
+    RK4 accumulator, output and (last stage) output residue, plus per stage
+    the FP32 phi_ref*sqrtG, reference gradient (2 words) and Coriolis
+    parameter."""
     if scheme == "RK4":
-        # in 4x12, base hi 3x12, base lo 4x12, acc 12+24+24+12, out 4x12 + lo 12,
-        # static 4 x (4 + 16 + 8)
-        return 48 + 36 + 48 + 72 + 48 + 12 + 4 * 28
-    # RK3: in 3x12, base hi 2x12, base lo 3x12, out 3x12 + lo 12, static 3 x 28
-    return 36 + 24 + 36 + 36 + 12 + 3 * 28
+        # in 4x12, hi 3x12, lo 12, acc 12+24+24+12, out 4x12 + lo 12, static 4 x 16
+        return 48 + 36 + 12 + 72 + 48 + 12 + 4 * 16
+    # RK3: in 3x12, hi 2x12, lo 12, out 3x12 + lo 12, static 3 x 16
+    return 36 + 24 + 12 + 36 + 12 + 3 * 16
 
 
 def measured_traffic(path):
@@ -200,6 +205,47 @@ def run_ensemble(capi, hostlib, stages, warmup_stages):
             "members_done_steps": int(done[:E_MEMBERS].min())}
 
 
+def run_serve(capi, seconds, rate=20.0, seed=0, check=6):
+    """The serving workload: Poisson arrivals at `rate` req/s for `seconds`,
+    one seeded solve per request (order, size, length and precision drawn as
+    configured), continuous batching with admission by HBM footprint.  After
+    the run, up to `check` of the smallest FP32C requests are re-run alone in
+    FP64 and their h/u/v distance reported."""
+    from mcv_sw_amd import hostlib, serve
+
+    reqs = serve.draw_requests(rate, seconds, seed=seed)
+    srv = serve.Server(capacity=256, hbm_budget=200 << 30, stages_per_poll=12, workers=16)
+    rep, res = srv.run(reqs, keep=("phi", "u", "v"))
+    errs = []
+    sample = sorted((r for r in reqs if r.precision == capi.FP32C and r.rid in res),
+                    key=lambda r: r.cells)[:check]
+    for r in sample:
+        m = hostlib.mesh(r.dof, r.n)
+        st = hostlib.jet(m, u0=r.u0, h0=r.h0, lmax=r.lmax, seed=r.seed, rms_h=r.rms_h)
+        with capi.Solver(r.dof, r.n, r.dt, m, st, scheme=r.scheme) as s:
+            s.step(r.nsteps)
+            ref = s.sync(fields=("phi", "u", "v"))
+        e = distance(ref, res[r.rid], r.dof, r.n, ref["mass"])
+        errs.append({"rid": r.rid, "dof": r.dof, "n": r.n, "hours": round(r.hours, 2),
+                     "h_rel_l2": e["h_rel_l2"], "u_rel_l2": e["u_rel_l2"],
+                     "v_rel_l2": e["v_rel_l2"]})
+    sizes = {}
+    for r in reqs:
+        sizes[f"mcv{r.dof}_n{r.n}"] = sizes.get(f"mcv{r.dof}_n{r.n}", 0) + 1
+    out = {k: v for k, v in rep.items() if k != "per_request"}
+    out["non_finite_requests"] = [
+        {k: x[k] for k in ("rid", "dof", "n", "precision", "hours", "u0", "rms_h")}
+        for x in rep["per_request"] if x["status"] != "ok"]
+    out.update({"workload": f"Poisson {rate} req/s for {seconds} s of arrivals; MCV3/MCV4 "
+                            f"p=0.5; n from {{32,64,128,256,384}} w 0.3/0.3/0.2/0.15/0.05; "
+                            f"6-48 simulated hours; FP64/FP32C p=0.5; RK4, CFL dt",
+                "requests_by_size": sizes,
+                "fp32c_error_vs_fp64_alone": errs,
+                "note": "latency = arrival to read-back of the finished solve, including "
+                        "host preparation of the initial state"})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +254,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
+                    help="seconds of Poisson arrivals for the serving run (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -286,6 +334,10 @@ def main():
     if not args.no_batch:
         res["batched"] = run_ensemble(capi, hostlib, stages=3 * 60, warmup_stages=3 * 4)
         log(f"ensemble: {res['batched']['value']:.3e} cell-steps/s")
+    if args.serve_seconds > 0:
+        res["serve"] = run_serve(capi, args.serve_seconds)
+        log(f"serve: {res['serve']['completed']} solves, p50 {res['serve']['latency_p50_s']:.2f} s, "
+            f"p99 {res['serve']['latency_p99_s']:.2f} s")
     cp_ms = capi.copy_probe(1 << 30, reps=10)
     res["roofline"]["copy_kernel_GBs"] = 2 * (1 << 30) / (cp_ms / 1e3) / 1e9
     if not args.no_cpu_baseline:

@@ -153,7 +153,8 @@ int mcv_hip_device_bytes(const mcv_handle *h, int64_t *bytes);
  * process: MCV_SW.F90:48-77).  A batch holds up to `capacity` solves of any
  * order, resolution, precision, scheme and time step on one device.  One
  * mcv_batch_run "stage" advances every solve in the batch by one RK stage of
- * its own scheme with one fused stage launch and one fused boundary launch;
+ * its own scheme with one fused stage launch and one fused boundary launch
+ * (plus a one-workgroup launch that advances the solves' stage counters);
  * solves admitted at different times run interleaved, each at its own
  * stage.  Each solve advances by exactly the nsteps given at admission and
  * then idles until removed.  Results are bit-identical to the same solve run

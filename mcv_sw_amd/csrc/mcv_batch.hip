@@ -2,7 +2,7 @@
 //
 // A batch holds up to `capacity` solves (slots) of any order (MCV3/MCV4),
 // resolution, precision (FP64/FP32C), RK scheme (RK3_TVD/RK4) and time step.
-// Each launch pair advances every listed solve by ONE RK stage:
+// Each stage (three launches) advances every listed solve by ONE RK stage:
 //
 //   batch_stage_kernel     persistent; walks the concatenated (solve, panel,
 //                          tile) work list of all listed solves, each at its
@@ -10,13 +10,13 @@
 //                          on the device, so solves admitted at different
 //                          times interleave freely)
 //   batch_boundary_kernel  persistent; the boundary pass of every solve's
-//                          freshly written level; the last workgroup to
-//                          finish advances every solve's stage index and
-//                          step counter
+//                          freshly written level
+//   batch_advance_kernel   one workgroup; advances every solve's stage index
+//                          and step counters
 //
 // All per-solve data the kernels need sits in device tables (solve
 // descriptors, stage state, work prefix sums), so one captured hipGraph of
-// kGraphStages launch pairs serves every batch composition: admitting or
+// kGraphStages stages serves every batch composition: admitting or
 // retiring a solve rewrites the tables between replays and never re-captures.
 // Solves whose step budget is used up are skipped by the kernels until the
 // host retires them.
@@ -36,7 +36,7 @@ namespace mcv {
 
 constexpr int kMaxStages = 4;
 constexpr int kBndChunk = 256;   // boundary items per work unit
-constexpr int kGraphStages = 12; // launch pairs per graph (a multiple of 3 and 4)
+constexpr int kGraphStages = 12; // stages per graph (a multiple of 3 and 4)
 constexpr int kXcds = 8;
 
 // Static description of one solve, written once at admission.
@@ -67,9 +67,9 @@ struct BatchSmem {
   static constexpr size_t a = TileLayout<double, 3, TileSet<3>::cx[V], TileSet<3>::cy[V]>::kBytes;
   static constexpr size_t b = TileLayout<double, 4, TileSet<4>::cx[V], TileSet<4>::cy[V]>::kBytes;
   static constexpr size_t c =
-      TileLayout<float, 3, TileSet32<3>::cx[V], TileSet32<3>::cy[V]>::kBytes;
+      TileLayout<FP32CStencil, 3, TileSet32<3>::cx[V], TileSet32<3>::cy[V]>::kBytes;
   static constexpr size_t d =
-      TileLayout<float, 4, TileSet32<4>::cx[V], TileSet32<4>::cy[V]>::kBytes;
+      TileLayout<FP32CStencil, 4, TileSet32<4>::cx[V], TileSet32<4>::cy[V]>::kBytes;
   static constexpr size_t ab = a > b ? a : b, cd = c > d ? c : d;
   static constexpr size_t kBytes = ab > cd ? ab : cd;
 };
@@ -162,12 +162,11 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
   }
 }
 
-// Persistent boundary kernel; the last workgroup to finish advances the
-// progress of every listed solve (stage index, then step counters).
+// Persistent boundary kernel: the boundary pass of every listed solve's
+// freshly written level, in work units of kBndChunk items.
 __global__ __launch_bounds__(256) void batch_boundary_kernel(
-    const BatchSolve *__restrict__ solves, BatchProgress *__restrict__ prog,
-    const int *__restrict__ list, const int *__restrict__ bprefix, const int *__restrict__ meta,
-    unsigned *done) {
+    const BatchSolve *__restrict__ solves, const BatchProgress *__restrict__ prog,
+    const int *__restrict__ list, const int *__restrict__ bprefix, const int *__restrict__ meta) {
   const int cnt = meta[0];
   const int total = cnt ? bprefix[cnt] : 0;
   int k = -1, kend = 0;
@@ -184,30 +183,28 @@ __global__ __launch_bounds__(256) void batch_boundary_kernel(
     const int base = (w - bprefix[k]) * kBndChunk;
     const int lim = min(kBndChunk, S.nbitems - base);
     if (S.prec == MCV_FP64) {
-      const MeshDev m = S.mesh;
       const BoundaryArgs b = S.b64[ph];
       for (int j = threadIdx.x; j < lim; j += blockDim.x) {
-        if (S.dof == 3) boundary_item<3>(m, b, base + j);
-        else boundary_item<4>(m, b, base + j);
+        if (S.dof == 3) boundary_item<3>(S.mesh, b, base + j);
+        else boundary_item<4>(S.mesh, b, base + j);
       }
     } else {
-      const MeshDev m = S.mesh;
       const BoundaryArgs32 b = S.b32[ph];
       for (int j = threadIdx.x; j < lim; j += blockDim.x) {
-        if (S.dof == 3) boundary_item_f32<3>(m, b, base + j);
-        else boundary_item_f32<4>(m, b, base + j);
+        if (S.dof == 3) boundary_item_f32<3>(S.mesh, b, base + j);
+        else boundary_item_f32<4>(S.mesh, b, base + j);
       }
     }
   }
-  __shared__ int last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+// Advances every listed solve by one stage (stage index, then step counters);
+// one workgroup, after the boundary kernel of the stage.
+__global__ __launch_bounds__(256) void batch_advance_kernel(const BatchSolve *__restrict__ solves,
+                                                            BatchProgress *__restrict__ prog,
+                                                            const int *__restrict__ list,
+                                                            const int *__restrict__ meta) {
+  const int cnt = meta[0];
   for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
     const int sl = list[e];
     BatchProgress st = prog[sl];
@@ -220,7 +217,6 @@ __global__ __launch_bounds__(256) void batch_boundary_kernel(
     }
     prog[sl] = st;
   }
-  if (threadIdx.x == 0) *done = 0u;  // ready for the next launch
 }
 
 }  // namespace mcv
@@ -237,7 +233,6 @@ struct mcv_batch {
   BatchSolve *d_solve = nullptr;
   BatchProgress *d_prog = nullptr;
   int *d_list = nullptr, *d_wprefix = nullptr, *d_bprefix = nullptr, *d_meta = nullptr;
-  unsigned *d_done = nullptr;
   std::vector<void *> allocs;
   int variant = 0;
   bool tables_dirty = true;
@@ -298,7 +293,7 @@ BatchSolve describe(const mcv_handle *h) {
   BatchSolve s{};
   s.dof = h->cfg.dof;
   s.prec = h->cfg.precision;
-  s.nbitems = 6 * 4 * h->npv + 8;
+  s.nbitems = boundary_items(h->cfg.dof, h->npv);
   s.mesh = h->mesh;
   s.mesh32 = h->mesh32;
   const double dt = h->cfg.dt;
@@ -383,7 +378,10 @@ int launch_stage(mcv_batch *b, int v) {
 
 int launch_boundary(mcv_batch *b) {
   hipLaunchKernelGGL(batch_boundary_kernel, dim3(b->grid_bnd), dim3(256), 0, b->stream,
-                     b->d_solve, b->d_prog, b->d_list, b->d_bprefix, b->d_meta, b->d_done);
+                     b->d_solve, b->d_prog, b->d_list, b->d_bprefix, b->d_meta);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(256), 0, b->stream, b->d_solve,
+                     b->d_prog, b->d_list, b->d_meta);
   HIP_TRY(hipGetLastError());
   return MCV_OK;
 }
@@ -457,20 +455,18 @@ int mcv_batch_create(int32_t device, int32_t capacity, mcv_batch **out) {
     int rc;
     if ((rc = balloc(b, &b->d_solve, capacity)) || (rc = balloc(b, &b->d_prog, capacity)) ||
         (rc = balloc(b, &b->d_list, capacity)) || (rc = balloc(b, &b->d_wprefix, capacity + 1)) ||
-        (rc = balloc(b, &b->d_bprefix, capacity + 1)) || (rc = balloc(b, &b->d_meta, 4)) ||
-        (rc = balloc(b, &b->d_done, 4)))
+        (rc = balloc(b, &b->d_bprefix, capacity + 1)) || (rc = balloc(b, &b->d_meta, 4)))
       return rc;
     HIP_TRY(hipMemcpy(b->d_prog, b->progress.data(), sizeof(BatchProgress) * capacity,
                       hipMemcpyHostToDevice));
-    const unsigned zero[4] = {0, 0, 0, 0};
-    HIP_TRY(hipMemcpy(b->d_done, zero, sizeof(zero), hipMemcpyHostToDevice));
+    const int zero[4] = {0, 0, 0, 0};
     HIP_TRY(hipMemcpy(b->d_meta, zero, sizeof(zero), hipMemcpyHostToDevice));
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     if ((rc = grid_for<0>(b, cus)) || (rc = grid_for<1>(b, cus)) || (rc = grid_for<2>(b, cus)) ||
         (rc = grid_for<3>(b, cus)))
       return rc;
-    b->grid_bnd = std::max(1, cus * 2);
+    b->grid_bnd = std::max(1, cus * 4);
     return MCV_OK;
   };
   int rc = init();

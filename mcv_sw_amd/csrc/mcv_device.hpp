@@ -97,6 +97,13 @@ __host__ __device__ inline long long edge_slot(int npv, int p, int side, int fie
   return ((long long)((p - 1) * 4 + side) * 3 + field) * npv + (s - 1);
 }
 
+// Work items of the boundary pass on one level: every ghost point (6 panels
+// x 4 sides x H depths x nPV), every point of the 12 cube edges (its two
+// copies averaged; the end points are skipped), and the 8 cube vertices.
+__host__ __device__ inline int boundary_items(int dof, int npv) {
+  return 6 * 4 * (dof - 1) * npv + 12 * npv + 8;
+}
+
 // (i, j) of point s (1..nPV) along side `side` of a panel.
 __host__ __device__ inline void edge_ij(int npv, int side, int s, int &i, int &j) {
   if (side == 0) { i = 1; j = s; }
@@ -138,11 +145,6 @@ static __constant__ EdgePair kEdges[12] = {
     {1, 0, 4, 1, 0}, {2, 0, 1, 1, 0}, {3, 0, 2, 1, 0}, {4, 0, 3, 1, 0},
     {1, 3, 5, 2, 0}, {1, 2, 6, 3, 0}, {2, 3, 5, 1, 0}, {2, 2, 6, 1, 1},
     {3, 3, 5, 3, 1}, {3, 2, 6, 2, 1}, {4, 3, 5, 0, 1}, {4, 2, 6, 0, 0},
-};
-// index into kEdges of the edge whose first panel side is (panel, side), or -1
-static __constant__ int8_t kFirstEdge[6][4] = {
-    {0, -1, 5, 4}, {1, -1, 7, 6}, {2, -1, 9, 8}, {3, -1, 11, 10}, {-1, -1, -1, -1},
-    {-1, -1, -1, -1},
 };
 static __constant__ VtxPt kVerts[8][3] = {
     {{1, 0, 1}, {4, 1, 1}, {5, 0, 0}}, {{1, 1, 1}, {2, 0, 1}, {5, 1, 0}},

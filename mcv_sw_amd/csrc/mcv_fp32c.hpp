@@ -25,13 +25,13 @@ namespace mcv {
 // Static data of an FP32C solve (per resolution and reference state).
 struct MeshF32 {
   int dof, n, npv, nh;
-  float rdx, r2dx, rdxx;        // 1/dx, 1/(2 dx), 1/dx^2
-  float dxf;                    // dx
+  double rdx, r2dx, rdxx;       // 1/dx, 1/(2 dx), 1/dx^2
+  double dxf;                   // dx
   long long metric_pstride;
   const double *sg, *rsg, *ig11, *ig12, *ig22;  // FP64 metric (same arrays as the FP64 path)
-  const double *f;              // Coriolis parameter (FP64)
+  const float *f;               // RN32(Coriolis parameter)
   const float *pgref;           // RN32(phi_ref * sqrtG)
-  const double *dEx, *dEy;      // MCV derivatives of phi_ref + phi_s (FP64, owned points)
+  const float *dEx, *dEy;       // RN32 of the FP64 MCV derivatives of phi_ref + phi_s
 };
 
 struct StageArgs32 {
@@ -46,51 +46,61 @@ struct StageArgs32 {
   double c1;                               // stage increment coefficient (x dt)
 };
 
-__device__ __forceinline__ float fit_left_f(const float *pv, int dof, float r, float r2) {
-  if (dof == 3) return -((pv[2] - 4.0f * pv[1] + 3.0f * pv[0]) * r);
-  return (-11.0f * pv[0] + 18.0f * pv[1] - 9.0f * pv[2] + 2.0f * pv[3]) * r2;
+// Arithmetic type of the FP32C stencil (LDS tiles, one-sided fits, Riemann
+// fluxes, interior derivatives); level storage is FP32 whatever this is.
+// FP32: an FP64 stencil on the same FP32 levels moves the distance from the
+// FP64 trajectory by less than 15% (the FP32 rounding of the stored stage
+// states dominates it) and doubles the LDS tile.
+using FP32CStencil = float;
+
+template <typename T>
+__device__ __forceinline__ T fit_left_t(const T *pv, int dof, T r, T r2) {
+  if (dof == 3) return -((pv[2] - T(4) * pv[1] + T(3) * pv[0]) * r);
+  return (-T(11) * pv[0] + T(18) * pv[1] - T(9) * pv[2] + T(2) * pv[3]) * r2;
 }
-__device__ __forceinline__ float fit_right_f(const float *pv, int dof, float r, float r2) {
-  if (dof == 3) return (pv[0] - 4.0f * pv[1] + 3.0f * pv[2]) * r;
-  return (11.0f * pv[3] - 18.0f * pv[2] + 9.0f * pv[1] - 2.0f * pv[0]) * r2;
+template <typename T>
+__device__ __forceinline__ T fit_right_t(const T *pv, int dof, T r, T r2) {
+  if (dof == 3) return (pv[0] - T(4) * pv[1] + T(3) * pv[2]) * r;
+  return (T(11) * pv[3] - T(18) * pv[2] + T(9) * pv[1] - T(2) * pv[0]) * r2;
 }
-__device__ __forceinline__ float riemann_f(float fxL, float fxR, float qxL, float qxR, float lam) {
-  return 0.5f * ((fxL + fxR) - lam * (qxR - qxL));
+template <typename T>
+__device__ __forceinline__ T riemann_t(T fxL, T fxR, T qxL, T qxR, T lam) {
+  return T(0.5) * ((fxL + fxR) - lam * (qxR - qxL));
 }
-template <int DOF>
-__device__ __forceinline__ float cell_interior_f(const float *f, float dL, float dR, int pos,
-                                                 float dxf, float rdx, float rdxx) {
+template <int DOF, typename T>
+__device__ __forceinline__ T cell_interior_t(const T *f, T dL, T dR, int pos, T dxf, T rdx,
+                                             T rdxx) {
   if constexpr (DOF == 3) {
-    return 1.5f * ((f[2] - f[0]) * rdx) - 0.25f * (dL + dR);
+    return T(1.5) * ((f[2] - f[0]) * rdx) - T(0.25) * (dL + dR);
   } else {
-    const float via = (f[3] - f[0]) * rdx;
-    const float fxx = 4.5f * (f[0] - f[1] - f[2] + f[3]) * rdxx;
+    const T via = (f[3] - f[0]) * rdx;
+    const T fxx = T(4.5) * (f[0] - f[1] - f[2] + f[3]) * rdxx;
     const bool first = pos == 1;
-    const float cl = first ? 4.0f : 5.0f, cr = first ? 5.0f : 4.0f;
-    const float a = (4.0f / 3.0f) * via - (cl * dL + cr * dR) * (1.0f / 27.0f);
-    const float c = (4.0f / 27.0f) * dxf * fxx;
+    const T cl = first ? T(4) : T(5), cr = first ? T(5) : T(4);
+    const T a = (T(4) / T(3)) * via - (cl * dL + cr * dR) * (T(1) / T(27));
+    const T c = (T(4) / T(27)) * dxf * fxx;
     return first ? a - c : a + c;
   }
 }
 
 // One (panel p, tile tx, ty) of the FP32C operator + RK stage `kind`; LDS of
-// at least TileLayout<float, DOF, CX, CY>::kBytes.
+// at least TileLayout<FP32CStencil, DOF, CX, CY>::kBytes.
 template <int DOF, int CX, int CY>
 __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs32 &a,
                                                const int kind, const int p, const int tx,
                                                const int ty, char *smem) {
-  using TL = TileLayout<float, DOF, CX, CY>;
+  using C = FP32CStencil;
+  using TL = TileLayout<C, DOF, CX, CY>;
   constexpr int H = DOF - 1;
   constexpr int NX = TL::NX, NY = TL::NY, TX = TL::TX, TY = TL::TY;
   const TL lds(smem);
-  float *const sE = lds.sE, *const sU = lds.sU, *const sV = lds.sV, *const sPG = lds.sPG,
-              *const sCU = lds.sCU, *const sCV = lds.sCV, *const lamx = lds.lamx,
-              *const lamy = lds.lamy, *const rxF = lds.rxF, *const rxD = lds.rxD,
-              *const rxV = lds.rxV, *const ryF = lds.ryF, *const ryD = lds.ryD,
-              *const ryU = lds.ryU;
+  C *const sE = lds.sE, *const sU = lds.sU, *const sV = lds.sV, *const sPG = lds.sPG,
+          *const sCU = lds.sCU, *const sCV = lds.sCV, *const lamx = lds.lamx,
+          *const lamy = lds.lamy, *const rxF = lds.rxF, *const rxD = lds.rxD,
+          *const rxV = lds.rxV, *const ryF = lds.ryF, *const ryD = lds.ryD, *const ryU = lds.ryU;
 
   const int npv = m.npv, nh = m.nh, N = m.n;
-  const float rdx = m.rdx, r2dx = m.r2dx, rdxx = m.rdxx, dxf = m.dxf;
+  const C rdx = (C)m.rdx, r2dx = (C)m.r2dx, rdxx = (C)m.rdxx, dxf = (C)m.dxf;
   const int cx0 = tx * CX, cy0 = ty * CY;
   const int ncx = min(CX, N - cx0), ncy = min(CY, N - cy0);
   const int i0 = cx0 * H + 1, j0 = cy0 * H + 1;
@@ -101,39 +111,62 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   const long long pbase = (long long)(p - 1) * nh * nh;
   const long long mbase = (long long)(p - 1) * m.metric_pstride;
 
-  for (int t = threadIdx.x; t < TX * TY; t += blockDim.x) {
-    const int ly = t / TX, lx = t - ly * TX;
-    if (lx >= lw || ly >= lh) continue;
-    const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
-    if (!inx && !iny) continue;
-    const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
-    const long long g = pbase + off;
-    const float dp = a.in_phi[g], u = a.in_u[g], v = a.in_v[g];
-    // metric products in FP64 (the FP32 inputs are exact in FP64), rounded once
-    const double ud = u, vd = v;
-    const double g11 = m.ig11[mbase + off], g12 = m.ig12[mbase + off],
-                 g22 = m.ig22[mbase + off];
-    const double cud = g11 * ud + g12 * vd, cvd = g12 * ud + g22 * vd;
-    const double Kd = 0.5 * (cud * ud + cvd * vd);
-    const float pg = (float)((double)m.pgref[g] + (double)dp * m.sg[mbase + off]);
-    const float cu = (float)cud, cv = (float)cvd;
-    const int li = ly * TX + lx;
-    sE[li] = (float)((double)dp + Kd);  // E - E_ref
-    sU[li] = u;
-    sV[li] = v;
-    sPG[li] = pg;
-    sCU[li] = cu;
-    sCV[li] = cv;
-    if (inx && iny) {
-      const int ox = lx - H, oy = ly - H;
-      const float phi = (float)((double)pg * m.rsg[mbase + off]);
-      if (ox % H == 0) {
-        const float s = sqrtf((float)g11 * phi);
-        lamx[(ox / H) * NY + oy] = fmaxf(fabsf(cu - s), fabsf(cu + s));
-      }
-      if (oy % H == 0) {
-        const float s = sqrtf((float)g22 * phi);
-        lamy[(oy / H) * NX + ox] = fmaxf(fabsf(cv - s), fabsf(cv + s));
+  // loads of all a thread's points first, then the derived values
+  constexpr int kP0 = (TX * TY + kStageThreads - 1) / kStageThreads;
+  for (int t0 = threadIdx.x; t0 < TX * TY; t0 += kP0 * blockDim.x) {
+    float rdp[kP0], ru[kP0], rv[kP0], rpg[kP0];
+    double r11[kP0], r12[kP0], r22[kP0], rsg[kP0], rrs[kP0];
+    bool live[kP0];
+#pragma unroll
+    for (int q = 0; q < kP0; ++q) {
+      const int t = t0 + q * blockDim.x;
+      const int ly = t / TX, lx = t - ly * TX;
+      const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
+      live[q] = t < TX * TY && lx < lw && ly < lh && (inx || iny);
+      if (!live[q]) continue;
+      const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
+      const long long g = pbase + off;
+      rdp[q] = a.in_phi[g];
+      ru[q] = a.in_u[g];
+      rv[q] = a.in_v[g];
+      rpg[q] = m.pgref[g];
+      r11[q] = m.ig11[mbase + off];
+      r12[q] = m.ig12[mbase + off];
+      r22[q] = m.ig22[mbase + off];
+      rsg[q] = m.sg[mbase + off];
+      rrs[q] = m.rsg[mbase + off];
+    }
+#pragma unroll
+    for (int q = 0; q < kP0; ++q) {
+      if (!live[q]) continue;
+      const int li = t0 + q * blockDim.x;  // = ly * TX + lx
+      const int ly = li / TX, lx = li - ly * TX;
+      const float dp = rdp[q], u = ru[q], v = rv[q];
+      // metric products in FP64 (the FP32 inputs are exact in FP64), rounded once
+      const double ud = u, vd = v;
+      const double g11 = r11[q], g12 = r12[q], g22 = r22[q];
+      const double cud = g11 * ud + g12 * vd, cvd = g12 * ud + g22 * vd;
+      const double Kd = 0.5 * (cud * ud + cvd * vd);
+      const C pg = (C)((double)rpg[q] + (double)dp * rsg[q]);
+      const C cu = (C)cud, cv = (C)cvd;
+      sE[li] = (C)((double)dp + Kd);  // E - E_ref
+      sU[li] = u;
+      sV[li] = v;
+      sPG[li] = pg;
+      sCU[li] = cu;
+      sCV[li] = cv;
+      const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
+      if (inx && iny) {
+        const int ox = lx - H, oy = ly - H;
+        const C phi = (C)((double)pg * rrs[q]);
+        if (ox % H == 0) {
+          const C sq = sqrt((C)g11 * phi);
+          lamx[(ox / H) * NY + oy] = fmax(fabs(cu - sq), fabs(cu + sq));
+        }
+        if (oy % H == 0) {
+          const C sq = sqrt((C)g22 * phi);
+          lamy[(oy / H) * NX + ox] = fmax(fabs(cv - sq), fabs(cv + sq));
+        }
       }
     }
   }
@@ -149,11 +182,10 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
     if (isx ? (mb > ncx || o >= ny) : (mb > ncy || o >= nx)) continue;
     const int st = isx ? 1 : TX;
     const int b = isx ? (o + H) * TX + (H + mb * H) : (H + mb * H) * TX + (o + H);
-    const float *sC = isx ? sCU : sCV;
-    const float *sQ = isx ? sU : sV;
-    const float *sW = isx ? sV : sU;
-    float eL[DOF], eR[DOF], qL[DOF], qR[DOF], dL[DOF], dR[DOF], pL[DOF], pR[DOF], wL[DOF],
-        wR[DOF];
+    const C *sC = isx ? sCU : sCV;
+    const C *sQ = isx ? sU : sV;
+    const C *sW = isx ? sV : sU;
+    C eL[DOF], eR[DOF], qL[DOF], qR[DOF], dL[DOF], dR[DOF], pL[DOF], pR[DOF], wL[DOF], wR[DOF];
 #pragma unroll
     for (int k = 0; k < DOF; ++k) {
       const int l = b - H * st + k * st, r = b + k * st;
@@ -164,17 +196,15 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       dR[k] = sPG[r] * sC[r];
       wL[k] = sW[l];  wR[k] = sW[r];
     }
-    const float lam = isx ? lamx[tt] : lamy[tt];
-    const float c = sC[b];
-    const float up = c == 0.0f ? 1.0f : copysignf(1.0f, c);
-    const float fF = riemann_f(fit_right_f(eL, DOF, rdx, r2dx), fit_left_f(eR, DOF, rdx, r2dx),
-                               fit_right_f(qL, DOF, rdx, r2dx), fit_left_f(qR, DOF, rdx, r2dx),
-                               lam);
-    const float fD = riemann_f(fit_right_f(dL, DOF, rdx, r2dx), fit_left_f(dR, DOF, rdx, r2dx),
-                               fit_right_f(pL, DOF, rdx, r2dx), fit_left_f(pR, DOF, rdx, r2dx),
-                               lam);
-    const float wlr = fit_right_f(wL, DOF, rdx, r2dx), wrl = fit_left_f(wR, DOF, rdx, r2dx);
-    const float fW = riemann_f(wlr, wrl, wlr, wrl, up);
+    const C lam = isx ? lamx[tt] : lamy[tt];
+    const C c = sC[b];
+    const C up = c == C(0) ? C(1) : copysign(C(1), c);
+    const C fF = riemann_t(fit_right_t(eL, DOF, rdx, r2dx), fit_left_t(eR, DOF, rdx, r2dx),
+                           fit_right_t(qL, DOF, rdx, r2dx), fit_left_t(qR, DOF, rdx, r2dx), lam);
+    const C fD = riemann_t(fit_right_t(dL, DOF, rdx, r2dx), fit_left_t(dR, DOF, rdx, r2dx),
+                           fit_right_t(pL, DOF, rdx, r2dx), fit_left_t(pR, DOF, rdx, r2dx), lam);
+    const C wlr = fit_right_t(wL, DOF, rdx, r2dx), wrl = fit_left_t(wR, DOF, rdx, r2dx);
+    const C fW = riemann_t(wlr, wrl, wlr, wrl, up);
     if (isx) {
       rxF[tt] = fF; rxD[tt] = fD; rxV[tt] = fW;
     } else {
@@ -183,144 +213,200 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   }
   __syncthreads();
 
+  // global loads of all a thread's points first, then their LDS work
   constexpr int kCells = CX * CY, kSlots = H * H * kCells, kTail = NY + NX - 1;
-  for (int t = threadIdx.x; t < kSlots + kTail; t += blockDim.x) {
-    int ox, oy;
-    if (t < kSlots) {
-      const int cls = t / kCells, cell = t - cls * kCells;
-      const int cyl = cell / CX, cxl = cell - cyl * CX;
-      if (cxl >= ncx || cyl >= ncy) continue;
-      ox = cxl * H + cls % H;
-      oy = cyl * H + cls / H;
-    } else if (t < kSlots + NY) {
-      oy = t - kSlots;
-      ox = ncx * H;
-      if (ox >= TXo || oy >= TYo) continue;
-    } else {
-      ox = t - kSlots - NY;
-      oy = ncy * H;
-      if (ox >= ncx * H || oy >= TYo) continue;
-    }
-    const int lx = ox + H, ly = oy + H, li = ly * TX + lx;
-    float fx, dvx, vx, fy, dvy, uy;
-    {
-      const int mb = ox / H, pos = ox - mb * H;
-      const int r0 = mb * NY + oy;
-      if (pos == 0) {
-        fx = rxF[r0]; dvx = rxD[r0]; vx = rxV[r0];
-      } else {
-        const int b0 = ly * TX + H + mb * H;
-        float fe[DOF], fd[DOF], fv[DOF];
+  constexpr int kP2 = (kSlots + kTail + kStageThreads - 1) / kStageThreads;
+  const bool first = kind == kRK4S1 || kind == kRK3S1;
+  const bool need_in = first || kind == kRK3S2 || kind == kRK3S3;
+  const bool need_acc = kind == kRK4S2 || kind == kRK4S3 || kind == kRK4S4;
+  // Only the step's last stage needs Y_n to full (hi + lo) precision: the
+  // intermediate stage states are rounded to FP32 anyway.
+  const bool last = kind == kRK4S4 || kind == kRK3S3;
+  for (int t0 = threadIdx.x; t0 < kSlots + kTail; t0 += kP2 * blockDim.x) {
+    int rox[kP2], roy[kP2];
+    bool live[kP2];
+    double rsg[kP2], rrs[kP2];
+    float rf[kP2], rdx_[kP2], rdy_[kP2], rin[kP2], rbp[kP2], rbu[kP2], rbv[kP2], rcp[kP2],
+        rcu[kP2], rcv[kP2], rap[kP2], rau[kP2], rav[kP2];
 #pragma unroll
-        for (int k = 0; k < DOF; ++k) {
-          fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
-        }
-        const int r1 = r0 + NY;
-        fx = cell_interior_f<DOF>(fe, rxF[r0], rxF[r1], pos, dxf, rdx, rdxx);
-        dvx = cell_interior_f<DOF>(fd, rxD[r0], rxD[r1], pos, dxf, rdx, rdxx);
-        vx = cell_interior_f<DOF>(fv, rxV[r0], rxV[r1], pos, dxf, rdx, rdxx);
+    for (int q = 0; q < kP2; ++q) {
+      const int t = t0 + q * blockDim.x;
+      int ox = 0, oy = 0;
+      bool ok = false;
+      if (t < kSlots) {
+        const int cls = t / kCells, cell = t - cls * kCells;
+        const int cyl = cell / CX, cxl = cell - cyl * CX;
+        ok = cxl < ncx && cyl < ncy;
+        ox = cxl * H + cls % H;
+        oy = cyl * H + cls / H;
+      } else if (t < kSlots + NY) {
+        oy = t - kSlots;
+        ox = ncx * H;
+        ok = ox < TXo && oy < TYo;
+      } else if (t < kSlots + kTail) {
+        ox = t - kSlots - NY;
+        oy = ncy * H;
+        ok = ox < ncx * H && oy < TYo;
+      }
+      live[q] = ok;
+      rox[q] = ox;
+      roy[q] = oy;
+      if (!ok) continue;
+      const long long off = (long long)(j0 + oy + H - 1) * nh + (i0 + ox + H - 1);
+      const long long g = pbase + off;
+      rsg[q] = m.sg[mbase + off];
+      rrs[q] = m.rsg[mbase + off];
+      rf[q] = m.f[g];
+      rdx_[q] = m.dEx[g];
+      rdy_[q] = m.dEy[g];
+      if (kind == kTend) continue;
+      if (last) {
+        rcp[q] = a.c_phi[g];
+        rcu[q] = a.c_u[g];
+        rcv[q] = a.c_v[g];
+      }
+      if (need_in) rin[q] = a.in_phi[g];
+      if (!first) {
+        rbp[q] = a.b_phi[g];
+        rbu[q] = a.b_u[g];
+        rbv[q] = a.b_v[g];
+      }
+      if (need_acc) {
+        rap[q] = a.acc_phiG[g];
+        rau[q] = a.acc_u[g];
+        rav[q] = a.acc_v[g];
       }
     }
-    {
-      const int mb = oy / H, pos = oy - mb * H;
-      const int r0 = mb * NX + ox;
-      if (pos == 0) {
-        fy = ryF[r0]; dvy = ryD[r0]; uy = ryU[r0];
-      } else {
-        const int b0 = (H + mb * H) * TX + lx;
-        float fe[DOF], fd[DOF], fu[DOF];
 #pragma unroll
-        for (int k = 0; k < DOF; ++k) {
-          const int l = b0 + k * TX;
-          fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
-        }
-        const int r1 = r0 + NX;
-        fy = cell_interior_f<DOF>(fe, ryF[r0], ryF[r1], pos, dxf, rdx, rdxx);
-        dvy = cell_interior_f<DOF>(fd, ryD[r0], ryD[r1], pos, dxf, rdx, rdxx);
-        uy = cell_interior_f<DOF>(fu, ryU[r0], ryU[r1], pos, dxf, rdx, rdxx);
-      }
-    }
-    const int i = i0 + ox, j = j0 + oy;
-    const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
-    const long long g = pbase + off;
-    // tendencies assembled in FP64 (spatial_operators_mod.F90:127-131)
-    const double sg = m.sg[mbase + off], rsg = m.rsg[mbase + off];
-    const double fc = m.f[g];
-    const double cu = sCU[li], cv = sCV[li];
-    const double vort = ((double)vx - (double)uy) * rsg;
-    const double kpg = -(double)dvx - (double)dvy;
-    const double ku = -(m.dEx[g] + (double)fx) + sg * (vort + fc) * cv;
-    const double kv = -(m.dEy[g] + (double)fy) - sg * (vort + fc) * cu;
-    const double kph = kpg * rsg;  // d(phi')/dt = d(phiG)/dt / sqrtG
-    if (kind == kTend) {
-      const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
-      a.o_phi[ti] = (float)kpg;
-      a.o_u[ti] = (float)ku;
-      a.o_v[ti] = (float)kv;
-    } else {
-      // base state Y_n = hi + lo in FP64
-      double bph, bu, bv;
-      if (kind == kRK4S1 || kind == kRK3S1) {
-        bph = (double)a.in_phi[g] + (double)a.c_phi[g];
-        bu = (double)sU[li] + (double)a.c_u[g];
-        bv = (double)sV[li] + (double)a.c_v[g];
-      } else {
-        bph = (double)a.b_phi[g] + (double)a.c_phi[g];
-        bu = (double)a.b_u[g] + (double)a.c_u[g];
-        bv = (double)a.b_v[g] + (double)a.c_v[g];
-      }
-      double ph, u, v;
-      if (kind == kRK4S1) {
-        ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-        a.acc_phiG[g] = (float)kph; a.acc_u[g] = (float)ku; a.acc_v[g] = (float)kv;
-      } else if (kind == kRK4S2 || kind == kRK4S3) {
-        ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-        a.acc_phiG[g] = (float)((double)a.acc_phiG[g] + 2.0 * kph);
-        a.acc_u[g] = (float)((double)a.acc_u[g] + 2.0 * ku);
-        a.acc_v[g] = (float)((double)a.acc_v[g] + 2.0 * kv);
-      } else if (kind == kRK4S4) {
-        ph = bph + a.c1 * (((double)a.acc_phiG[g] + kph) * (1.0 / 6.0));
-        u = bu + a.c1 * (((double)a.acc_u[g] + ku) * (1.0 / 6.0));
-        v = bv + a.c1 * (((double)a.acc_v[g] + kv) * (1.0 / 6.0));
-      } else if (kind == kRK3S1) {
-        ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-      } else if (kind == kRK3S2) {  // Yn + 1/4 (Y1 - Yn) + 1/4 dt k
-        ph = bph + 0.25 * ((double)a.in_phi[g] - bph) + a.c1 * kph;
-        u = bu + 0.25 * ((double)sU[li] - bu) + a.c1 * ku;
-        v = bv + 0.25 * ((double)sV[li] - bv) + a.c1 * kv;
-      } else {  // kRK3S3: Yn + 2/3 (Y2 - Yn) + 2/3 dt k
-        ph = bph + (2.0 / 3.0) * ((double)a.in_phi[g] - bph) + a.c1 * kph;
-        u = bu + (2.0 / 3.0) * ((double)sU[li] - bu) + a.c1 * ku;
-        v = bv + (2.0 / 3.0) * ((double)sV[li] - bv) + a.c1 * kv;
-      }
-      const float hph = (float)ph, hu = (float)u, hv = (float)v;
-      a.o_phi[g] = hph;
-      a.o_u[g] = hu;
-      a.o_v[g] = hv;
-      if (a.oc_phi) {  // last stage: keep the rounding residue (Kahan word)
-        a.oc_phi[g] = (float)(ph - (double)hph);
-        a.oc_u[g] = (float)(u - (double)hu);
-        a.oc_v[g] = (float)(v - (double)hv);
-      }
-      if (i == 1 || i == npv || j == 1 || j == npv) {
-        const float vals[3] = {hph, hu, hv};
-        float lows[3] = {0.0f, 0.0f, 0.0f};
-        if (a.oc_phi) {
-          lows[0] = (float)(ph - (double)hph);
-          lows[1] = (float)(u - (double)hu);
-          lows[2] = (float)(v - (double)hv);
-        }
+    for (int q = 0; q < kP2; ++q) {
+      if (!live[q]) continue;
+      const int ox = rox[q], oy = roy[q];
+      const int lx = ox + H, ly = oy + H, li = ly * TX + lx;
+      C fx, dvx, vx, fy, dvy, uy;
+      {
+        const int mb = ox / H, pos = ox - mb * H;
+        const int r0 = mb * NY + oy;
+        if (pos == 0) {
+          fx = rxF[r0]; dvx = rxD[r0]; vx = rxV[r0];
+        } else {
+          const int b0 = ly * TX + H + mb * H;
+          C fe[DOF], fd[DOF], fv[DOF];
 #pragma unroll
-        for (int fl = 0; fl < 3; ++fl) {
-          long long sl[4];
-          int ns = 0;
-          if (i == 1) sl[ns++] = edge_slot(npv, p, 0, fl, j);
-          if (i == npv) sl[ns++] = edge_slot(npv, p, 1, fl, j);
-          if (j == 1) sl[ns++] = edge_slot(npv, p, 2, fl, i);
-          if (j == npv) sl[ns++] = edge_slot(npv, p, 3, fl, i);
-          for (int q = 0; q < ns; ++q) {
-            a.edge_raw[sl[q]] = vals[fl];
-            if (a.oc_phi) a.edge_raw_c[sl[q]] = lows[fl];
+          for (int k = 0; k < DOF; ++k) {
+            fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
+          }
+          const int r1 = r0 + NY;
+          fx = cell_interior_t<DOF>(fe, rxF[r0], rxF[r1], pos, dxf, rdx, rdxx);
+          dvx = cell_interior_t<DOF>(fd, rxD[r0], rxD[r1], pos, dxf, rdx, rdxx);
+          vx = cell_interior_t<DOF>(fv, rxV[r0], rxV[r1], pos, dxf, rdx, rdxx);
+        }
+      }
+      {
+        const int mb = oy / H, pos = oy - mb * H;
+        const int r0 = mb * NX + ox;
+        if (pos == 0) {
+          fy = ryF[r0]; dvy = ryD[r0]; uy = ryU[r0];
+        } else {
+          const int b0 = (H + mb * H) * TX + lx;
+          C fe[DOF], fd[DOF], fu[DOF];
+#pragma unroll
+          for (int k = 0; k < DOF; ++k) {
+            const int l = b0 + k * TX;
+            fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
+          }
+          const int r1 = r0 + NX;
+          fy = cell_interior_t<DOF>(fe, ryF[r0], ryF[r1], pos, dxf, rdx, rdxx);
+          dvy = cell_interior_t<DOF>(fd, ryD[r0], ryD[r1], pos, dxf, rdx, rdxx);
+          uy = cell_interior_t<DOF>(fu, ryU[r0], ryU[r1], pos, dxf, rdx, rdxx);
+        }
+      }
+      const int i = i0 + ox, j = j0 + oy;
+      const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
+      const long long g = pbase + off;
+      // tendencies assembled in FP64 (spatial_operators_mod.F90:127-131)
+      const double sg = rsg[q], rsgv = rrs[q];
+      const double fc = (double)rf[q];
+      const double cu = sCU[li], cv = sCV[li];
+      const double vort = ((double)vx - (double)uy) * rsgv;
+      const double kpg = -(double)dvx - (double)dvy;
+      const double ku = -((double)rdx_[q] + (double)fx) + sg * (vort + fc) * cv;
+      const double kv = -((double)rdy_[q] + (double)fy) - sg * (vort + fc) * cu;
+      const double kph = kpg * rsgv;  // d(phi')/dt = d(phiG)/dt / sqrtG
+      if (kind == kTend) {
+        const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
+        a.o_phi[ti] = (float)kpg;
+        a.o_u[ti] = (float)ku;
+        a.o_v[ti] = (float)kv;
+      } else {
+        // base state Y_n: hi (+ lo in the last stage) in FP64
+        double bph, bu, bv;
+        if (kind == kRK4S1 || kind == kRK3S1) {
+          bph = (double)rin[q];
+          bu = (double)sU[li];
+          bv = (double)sV[li];
+        } else {
+          bph = (double)rbp[q];
+          bu = (double)rbu[q];
+          bv = (double)rbv[q];
+        }
+        if (last) {
+          bph += (double)rcp[q];
+          bu += (double)rcu[q];
+          bv += (double)rcv[q];
+        }
+        double ph, u, v;
+        if (kind == kRK4S1) {
+          ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
+          a.acc_phiG[g] = (float)kph; a.acc_u[g] = (float)ku; a.acc_v[g] = (float)kv;
+        } else if (kind == kRK4S2 || kind == kRK4S3) {
+          ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
+          a.acc_phiG[g] = (float)((double)rap[q] + 2.0 * kph);
+          a.acc_u[g] = (float)((double)rau[q] + 2.0 * ku);
+          a.acc_v[g] = (float)((double)rav[q] + 2.0 * kv);
+        } else if (kind == kRK4S4) {
+          ph = bph + a.c1 * (((double)rap[q] + kph) * (1.0 / 6.0));
+          u = bu + a.c1 * (((double)rau[q] + ku) * (1.0 / 6.0));
+          v = bv + a.c1 * (((double)rav[q] + kv) * (1.0 / 6.0));
+        } else if (kind == kRK3S1) {
+          ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
+        } else if (kind == kRK3S2) {  // Yn + 1/4 (Y1 - Yn) + 1/4 dt k
+          ph = bph + 0.25 * ((double)rin[q] - bph) + a.c1 * kph;
+          u = bu + 0.25 * ((double)sU[li] - bu) + a.c1 * ku;
+          v = bv + 0.25 * ((double)sV[li] - bv) + a.c1 * kv;
+        } else {  // kRK3S3: Yn + 2/3 (Y2 - Yn) + 2/3 dt k
+          ph = bph + (2.0 / 3.0) * ((double)rin[q] - bph) + a.c1 * kph;
+          u = bu + (2.0 / 3.0) * ((double)sU[li] - bu) + a.c1 * ku;
+          v = bv + (2.0 / 3.0) * ((double)sV[li] - bv) + a.c1 * kv;
+        }
+        const float hph = (float)ph, hu = (float)u, hv = (float)v;
+        a.o_phi[g] = hph;
+        a.o_u[g] = hu;
+        a.o_v[g] = hv;
+        if (a.oc_phi) {  // last stage: keep the rounding residue (Kahan word)
+          a.oc_phi[g] = (float)(ph - (double)hph);
+          a.oc_u[g] = (float)(u - (double)hu);
+          a.oc_v[g] = (float)(v - (double)hv);
+        }
+        if (i == 1 || i == npv || j == 1 || j == npv) {
+          const float vals[3] = {hph, hu, hv};
+          float lows[3] = {0.0f, 0.0f, 0.0f};
+          if (a.oc_phi) {
+            lows[0] = (float)(ph - (double)hph);
+            lows[1] = (float)(u - (double)hu);
+            lows[2] = (float)(v - (double)hv);
+          }
+#pragma unroll
+          for (int fl = 0; fl < 3; ++fl) {
+            long long sl[4];
+            int ns = 0;
+            if (i == 1) sl[ns++] = edge_slot(npv, p, 0, fl, j);
+            if (i == npv) sl[ns++] = edge_slot(npv, p, 1, fl, j);
+            if (j == 1) sl[ns++] = edge_slot(npv, p, 2, fl, i);
+            if (j == npv) sl[ns++] = edge_slot(npv, p, 3, fl, i);
+            for (int e = 0; e < ns; ++e) {
+              a.edge_raw[sl[e]] = vals[fl];
+              if (a.oc_phi) a.edge_raw_c[sl[e]] = lows[fl];
+            }
           }
         }
       }
@@ -331,7 +417,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
 template <int DOF, int KIND, int CX, int CY>
 __global__ __launch_bounds__(kStageThreads) void stage_kernel_f32(MeshF32 m, StageArgs32 a,
                                                                   int tiles_x) {
-  __shared__ __align__(16) char smem[TileLayout<float, DOF, CX, CY>::kBytes];
+  __shared__ __align__(16) char smem[TileLayout<FP32CStencil, DOF, CX, CY>::kBytes];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = L / 6;
   stage_tile_f32<DOF, CX, CY>(m, a, KIND, L % 6 + 1, tile % tiles_x, tile / tiles_x, smem);
@@ -357,7 +443,7 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
   const float *const edge_raw = ba.edge_raw, *const edge_raw_c = ba.edge_raw_c;
   const int npv = m.npv, nh = m.nh;
   const double dx = m.dx;
-  const int nside = 6 * 4 * npv;
+  const int nghost = 6 * 4 * H * npv, nedge = 12 * npv;
   auto raw = [&](int q, int i, int j, double &ph, double &uu, double &vv) {
     int side = -1, s = 0;
     if (i == 1) { side = 0; s = j; }
@@ -387,10 +473,11 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
     }
   };
   {
-    if (t < nside) {
-      const int s = t % npv + 1, ps = t / npv, side = ps % 4, p = ps / 4 + 1;
+    if (t < nghost) {  // one ghost point: (panel, side, depth k, s), s fastest
+      const int s = t % npv + 1, r = t / npv, k = r % H + 1, ps = r / H;
+      const int side = ps % 4, p = ps / 4 + 1;
       const HaloSrc hs = kHalo[p - 1][side];
-      for (int k = 1; k <= H; ++k) {
+      {
         const int c = m.giref[(k - 1) * npv + (s - 1)];
         const double cf = m.gcoef[(k - 1) * npv + (s - 1)];
         const int nrm = hs.far ? npv - k : k + 1;
@@ -415,8 +502,9 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
         apply2(m.sp2p + 4 * g, zh, mh, uh, vh);
         store(g, ghost_interp<DOF>(qp, cf, dx), uh, vh);
       }
-      const int e = kFirstEdge[p - 1][side];
-      if (e >= 0 && s > 1 && s < npv) {
+    } else if (t < nghost + nedge) {  // one point of one cube edge
+      const int s = (t - nghost) % npv + 1, e = (t - nghost) / npv;
+      if (s > 1 && s < npv) {
         const EdgePair ep = kEdges[e];
         const int sb = ep.rev ? npv + 1 - s : s;
         int ia, ja, ib, jb;
@@ -437,7 +525,7 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
         store(gb, ph, u2, v2);
       }
     } else {
-      const int vx = t - nside;
+      const int vx = t - nghost - nedge;
       double ph[3], z[3], mr[3];
       long long gg[3];
       for (int l = 0; l < 3; ++l) {
@@ -462,9 +550,16 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
 
 template <int DOF>
 __global__ __launch_bounds__(256) void boundary_kernel_f32(MeshDev m, BoundaryArgs32 a) {
-  const int nitems = 6 * 4 * m.npv + 8;
+  const int nitems = boundary_items(DOF, m.npv);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nitems; t += gridDim.x * blockDim.x)
     boundary_item_f32<DOF>(m, a, t);
+}
+
+// RN32 copy of an FP64 array.
+static __global__ void round_to_f32_kernel(long long n, const double *x, float *y) {
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < n;
+       g += (long long)gridDim.x * blockDim.x)
+    y[g] = (float)x[g];
 }
 
 // pgref = RN32(phi_ref * sqrtG) on every point.
@@ -497,8 +592,8 @@ static __global__ void edge_copy_kernel(MeshDev m, const double *phi, const doub
 // dissipation term left out: the part of dE/dx, dE/dy that does not depend on
 // the evolving state.  One thread per owned point; runs once per solve.
 template <int DOF>
-__global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, double *dEx,
-                                    double *dEy) {
+__global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, float *dEx,
+                                    float *dEy) {
   constexpr int H = DOF - 1;
   const int npv = m.npv, nh = m.nh;
   const long long per = (long long)nh * nh;
@@ -529,8 +624,8 @@ __global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, double *dE
       return cell_interior<DOF>(f, dL, central(bL + H), pos, m.dx, m.dv);
     };
     const long long g = pidx(nh, H, p, i, j);
-    dEx[g] = deriv(true);
-    dEy[g] = deriv(false);
+    dEx[g] = (float)deriv(true);
+    dEy[g] = (float)deriv(false);
   }
 }
 

@@ -47,7 +47,7 @@ size_t footprint_bytes(const mcv_config *c) {
   if (!fp32c) b += 3 * d(np);                          // RK4 accumulator
   b += d(nedge) + 2 * d(np) + d(4 * diag_blocks) + d(2) + 6 * d(np);
   if (fp32c)
-    b += 3 * d(np) + 2 * d(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) + 2 * f(nedge) + f(np);
+    b += 3 * d(np) + 3 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) + 2 * f(nedge) + f(np);
   return b;
 }
 
@@ -194,7 +194,7 @@ int stage(mcv_handle *h, int kind, const StageArgs &a) {
 
 int boundary(mcv_handle *h, int level) {
   const BoundaryArgs b = boundary_args64(h, level);
-  const int threads = 6 * 4 * h->npv + 8;
+  const int threads = boundary_items(h->cfg.dof, h->npv);
   dim3 grid((threads + 255) / 256), block(256);
   if (h->cfg.dof == 3)
     hipLaunchKernelGGL(boundary_kernel<3>, grid, block, 0, h->stream, h->mesh, b);
@@ -249,7 +249,7 @@ int stage32(mcv_handle *h, int kind, const StageArgs32 &a) {
 
 int boundary32(mcv_handle *h, int level) {
   const BoundaryArgs32 b = boundary_args32(h, level);
-  const int threads = 6 * 4 * h->npv + 8;
+  const int threads = boundary_items(h->cfg.dof, h->npv);
   dim3 grid((threads + 255) / 256), block(256);
   if (h->cfg.dof == 3)
     hipLaunchKernelGGL(boundary_kernel_f32<3>, grid, block, 0, h->stream, h->mesh, b);
@@ -378,8 +378,13 @@ int setup32(mcv_handle *h) {
   HIP_TRY(hipGetLastError());
   if ((rc = boundary(h, 0))) return rc;
   h->first_pending = false;  // the FP32C stages form the contravariant winds themselves
-  double *dEx, *dEy;
-  if ((rc = dev_alloc(h, &dEx, np)) || (rc = dev_alloc(h, &dEy, np))) return rc;
+  float *dEx, *dEy, *f32;
+  if ((rc = dev_alloc_t(h, &dEx, np)) || (rc = dev_alloc_t(h, &dEy, np)) ||
+      (rc = dev_alloc_t(h, &f32, np)))
+    return rc;
+  hipLaunchKernelGGL(round_to_f32_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->mesh.f,
+                     f32);
+  HIP_TRY(hipGetLastError());
   if (h->cfg.dof == 3)
     hipLaunchKernelGGL(ref_gradient_kernel<3>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
                        h->lv[0][0], dEx, dEy);
@@ -415,17 +420,17 @@ int setup32(mcv_handle *h) {
   m.n = d.n;
   m.npv = d.npv;
   m.nh = d.nh;
-  m.rdx = (float)(1.0 / d.dx);
-  m.r2dx = (float)(1.0 / (2.0 * d.dx));
-  m.rdxx = (float)(1.0 / (d.dx * d.dx));
-  m.dxf = (float)d.dx;
+  m.rdx = 1.0 / d.dx;
+  m.r2dx = 1.0 / (2.0 * d.dx);
+  m.rdxx = 1.0 / (d.dx * d.dx);
+  m.dxf = d.dx;
   m.metric_pstride = d.metric_pstride;
   m.sg = d.sqrtG;
   m.rsg = d.rsqrtG;
   m.ig11 = d.ig11;
   m.ig12 = d.ig12;
   m.ig22 = d.ig22;
-  m.f = d.f;
+  m.f = f32;
   m.pgref = pgref;
   m.dEx = dEx;
   m.dEy = dEy;

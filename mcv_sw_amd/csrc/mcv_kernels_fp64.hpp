@@ -43,21 +43,25 @@ __device__ __forceinline__ void raw_point(const MeshDev &m, const BoundaryArgs &
 
 // correct_bdy_ghost (time_scheme_mod.F90:125-142) on one level: one thread per
 // (panel, side, s), plus 8 threads for the cube vertices.
-// Work item t of correct_bdy_ghost on one level, t in [0, 6*4*nPV + 8).
+// Work item t of correct_bdy_ghost on one level, t in [0, boundary_items()):
+// one ghost point, one edge point or one cube vertex per item, so that each
+// item is a short chain of loads and every item is independent.
 template <int DOF>
 __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryArgs &a,
                                               const int t) {
   constexpr int H = DOF - 1;
   const int npv = m.npv, nh = m.nh;
   const double dx = m.dx;
-  const int nside = 6 * 4 * npv;
-  if (t < nside) {
-    const int s = t % npv + 1, ps = t / npv, side = ps % 4, p = ps / 4 + 1;
+  const int nghost = 6 * 4 * H * npv, nedge = 12 * npv;
+  if (t < nghost) {
+    // (panel, side, depth k, along-edge s), s fastest
+    const int s = t % npv + 1, r = t / npv, k = r % H + 1, ps = r / H;
+    const int side = ps % 4, p = ps / 4 + 1;
     const HaloSrc hs = kHalo[p - 1][side];
-    // Ghost points at depths 1..H (ghost_mod.f90:154-227), sphere winds
-    // through P2SP at the neighbour's points and back through SP2P at the
-    // ghost point (spatial_operators_mod.F90:408-492).
-    for (int k = 1; k <= H; ++k) {
+    // Ghost point at depth k (ghost_mod.f90:154-227), sphere winds through
+    // P2SP at the neighbour's points and back through SP2P at the ghost point
+    // (spatial_operators_mod.F90:408-492).
+    {
       const int c = m.giref[(k - 1) * npv + (s - 1)];
       const double cf = m.gcoef[(k - 1) * npv + (s - 1)];
       const int nrm = hs.far ? npv - k : k + 1;
@@ -85,9 +89,10 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
       a.u[g] = uh;
       a.v[g] = vh;
     }
+  } else if (t < nghost + nedge) {
     // Edge averaging (spatial_operators_mod.F90:298-345), vertices excluded.
-    const int e = kFirstEdge[p - 1][side];
-    if (e >= 0 && s > 1 && s < npv) {
+    const int s = (t - nghost) % npv + 1, e = (t - nghost) / npv;
+    if (s > 1 && s < npv) {
       const EdgePair ep = kEdges[e];
       const int sb = ep.rev ? npv + 1 - s : s;
       const double pa = a.edge_raw[edge_slot(npv, ep.pa, ep.sa, 0, s)];
@@ -112,7 +117,7 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
     }
   } else {
     // Cube vertices (spatial_operators_mod.F90:347-386), reference summation order.
-    const int vx = t - nside;
+    const int vx = t - nghost - nedge;
     double ph[3], z[3], mr[3];
     long long gg[3];
 #pragma unroll
@@ -142,7 +147,7 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
 
 template <int DOF>
 __global__ __launch_bounds__(256) void boundary_kernel(MeshDev m, BoundaryArgs a) {
-  const int nitems = 6 * 4 * m.npv + 8;
+  const int nitems = boundary_items(DOF, m.npv);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nitems; t += gridDim.x * blockDim.x)
     boundary_item<DOF>(m, a, t);
 }

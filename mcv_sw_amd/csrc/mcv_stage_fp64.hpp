@@ -73,38 +73,64 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
   const long long sbase = (long long)(p - 1) * m.phis_pstride;
 
   // Phase 0: stage the cross-shaped tile + halo; derived fields per point.
-  for (int t = threadIdx.x; t < TX * TY; t += blockDim.x) {
-    const int ly = t / TX, lx = t - ly * TX;
-    if (lx >= lw || ly >= lh) continue;
-    const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
-    if (!inx && !iny) continue;  // halo corner: never read
-    const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
-    const long long g = pbase + off;
-    const double phi = a.in_phi[g], u = a.in_u[g], v = a.in_v[g];
-    const double sg = m.sqrtG[mbase + off];
-    const double g11 = m.ig11[mbase + off], g12 = m.ig12[mbase + off],
-                 g22 = m.ig22[mbase + off];
-    const double ps = m.phi_s[sbase + off];
-    double cu, cv;
-    if (a.cu0) {
-      cu = a.cu0[g];
-      cv = a.cv0[g];
-    } else {  // cov2contrav (projection_mod.F90:138-148)
-      cu = g11 * u + g12 * v;
-      cv = g12 * u + g22 * v;
+  // Every thread first issues the loads of all its points, then derives and
+  // stores, so that its loads are in flight together.
+  constexpr int kP0 = (TX * TY + kStageThreads - 1) / kStageThreads;
+  for (int t0 = threadIdx.x; t0 < TX * TY; t0 += kP0 * blockDim.x) {
+    double rphi[kP0], ru[kP0], rv[kP0], rsg[kP0], r11[kP0], r12[kP0], r22[kP0], rps[kP0];
+    double rcu[kP0], rcv[kP0];
+    bool live[kP0];
+#pragma unroll
+    for (int q = 0; q < kP0; ++q) {
+      const int t = t0 + q * blockDim.x;
+      const int ly = t / TX, lx = t - ly * TX;
+      const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
+      // outside the tile, or a halo corner (never read)
+      live[q] = t < TX * TY && lx < lw && ly < lh && (inx || iny);
+      if (!live[q]) continue;
+      const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
+      const long long g = pbase + off;
+      rphi[q] = a.in_phi[g];
+      ru[q] = a.in_u[g];
+      rv[q] = a.in_v[g];
+      rsg[q] = m.sqrtG[mbase + off];
+      r11[q] = m.ig11[mbase + off];
+      r12[q] = m.ig12[mbase + off];
+      r22[q] = m.ig22[mbase + off];
+      rps[q] = m.phi_s[sbase + off];
+      if (a.cu0) {
+        rcu[q] = a.cu0[g];
+        rcv[q] = a.cv0[g];
+      }
     }
-    const double K = 0.5 * (cu * u + cv * v);  // spatial_operators_mod.F90:60-61
-    const int li = ly * TX + lx;
-    sE[li] = phi + ps + K;
-    sU[li] = u;
-    sV[li] = v;
-    sPG[li] = phi * sg;
-    sCU[li] = cu;
-    sCV[li] = cv;
-    if (inx && iny) {
-      const int ox = lx - H, oy = ly - H;
-      if (ox % H == 0) lamx[(ox / H) * NY + oy] = wave_speed(cu, phi, g11);
-      if (oy % H == 0) lamy[(oy / H) * NX + ox] = wave_speed(cv, phi, g22);
+#pragma unroll
+    for (int q = 0; q < kP0; ++q) {
+      if (!live[q]) continue;
+      const int li = t0 + q * blockDim.x;  // = ly * TX + lx
+      const int ly = li / TX, lx = li - ly * TX;
+      const double phi = rphi[q], u = ru[q], v = rv[q], g11 = r11[q], g12 = r12[q],
+                   g22 = r22[q];
+      double cu, cv;
+      if (a.cu0) {
+        cu = rcu[q];
+        cv = rcv[q];
+      } else {  // cov2contrav (projection_mod.F90:138-148)
+        cu = g11 * u + g12 * v;
+        cv = g12 * u + g22 * v;
+      }
+      const double K = 0.5 * (cu * u + cv * v);  // spatial_operators_mod.F90:60-61
+      sE[li] = phi + rps[q] + K;
+      sU[li] = u;
+      sV[li] = v;
+      sPG[li] = phi * rsg[q];
+      sCU[li] = cu;
+      sCV[li] = cv;
+      const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
+      if (inx && iny) {
+        const int ox = lx - H, oy = ly - H;
+        if (ox % H == 0) lamx[(ox / H) * NY + oy] = wave_speed(cu, phi, g11);
+        if (oy % H == 0) lamy[(oy / H) * NX + ox] = wave_speed(cv, phi, g22);
+      }
     }
   }
   __syncthreads();
@@ -156,128 +182,166 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
   // fastest, so the lanes of a wave take the same branches and read LDS with
   // a stride of H points (no bank conflicts); the far cell-end column and row
   // owned by the last tiles follow.
+  // The global loads of all a thread's points are issued before their LDS
+  // work.
   constexpr int kCells = CX * CY, kSlots = H * H * kCells, kTail = NY + NX - 1;
-  for (int t = threadIdx.x; t < kSlots + kTail; t += blockDim.x) {
-    int ox, oy;
-    if (t < kSlots) {
-      const int cls = t / kCells, cell = t - cls * kCells;
-      const int cyl = cell / CX, cxl = cell - cyl * CX;
-      if (cxl >= ncx || cyl >= ncy) continue;
-      ox = cxl * H + cls % H;
-      oy = cyl * H + cls / H;
-    } else if (t < kSlots + NY) {
-      oy = t - kSlots;
-      ox = ncx * H;
-      if (ox >= TXo || oy >= TYo) continue;
-    } else {
-      ox = t - kSlots - NY;
-      oy = ncy * H;
-      if (ox >= ncx * H || oy >= TYo) continue;
-    }
-    const int lx = ox + H, ly = oy + H, li = ly * TX + lx;
-    double fx, dvx, vx, fy, dvy, uy;
-    {  // x sweep
-      const int mb = ox / H, pos = ox - mb * H;
-      const int r0 = mb * NY + oy;
-      if (pos == 0) {
-        fx = rxF[r0]; dvx = rxD[r0]; vx = rxV[r0];
-      } else {
-        const int b0 = ly * TX + H + mb * H;
-        double fe[DOF], fd[DOF], fv[DOF];
+  constexpr int kP2 = (kSlots + kTail + kStageThreads - 1) / kStageThreads;
+  const bool need_base = !(kind == kTend || kind == kRK4S1 || kind == kRK3S1);
+  const bool need_acc = kind == kRK4S2 || kind == kRK4S3 || kind == kRK4S4;
+  for (int t0 = threadIdx.x; t0 < kSlots + kTail; t0 += kP2 * blockDim.x) {
+    int rox[kP2], roy[kP2];
+    bool live[kP2];
+    double rsg[kP2], rrs[kP2], rf[kP2], rbp[kP2], rbu[kP2], rbv[kP2], rap[kP2], rau[kP2],
+        rav[kP2];
 #pragma unroll
-        for (int k = 0; k < DOF; ++k) {
-          fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
-        }
-        const int r1 = r0 + NY;
-        fx = cell_interior<DOF>(fe, rxF[r0], rxF[r1], pos, dx, dv);
-        dvx = cell_interior<DOF>(fd, rxD[r0], rxD[r1], pos, dx, dv);
-        vx = cell_interior<DOF>(fv, rxV[r0], rxV[r1], pos, dx, dv);
+    for (int q = 0; q < kP2; ++q) {
+      const int t = t0 + q * blockDim.x;
+      int ox = 0, oy = 0;
+      bool ok = false;
+      if (t < kSlots) {
+        const int cls = t / kCells, cell = t - cls * kCells;
+        const int cyl = cell / CX, cxl = cell - cyl * CX;
+        ok = cxl < ncx && cyl < ncy;
+        ox = cxl * H + cls % H;
+        oy = cyl * H + cls / H;
+      } else if (t < kSlots + NY) {
+        oy = t - kSlots;
+        ox = ncx * H;
+        ok = ox < TXo && oy < TYo;
+      } else if (t < kSlots + kTail) {
+        ox = t - kSlots - NY;
+        oy = ncy * H;
+        ok = ox < ncx * H && oy < TYo;
+      }
+      live[q] = ok;
+      rox[q] = ox;
+      roy[q] = oy;
+      if (!ok) continue;
+      const long long off = (long long)(j0 + oy + H - 1) * nh + (i0 + ox + H - 1);
+      const long long g = pbase + off;
+      rsg[q] = m.sqrtG[mbase + off];
+      rrs[q] = m.rsqrtG[mbase + off];
+      rf[q] = m.f[g];
+      if (need_base) {
+        rbp[q] = a.b_phi[g];
+        rbu[q] = a.b_u[g];
+        rbv[q] = a.b_v[g];
+      }
+      if (need_acc) {
+        rap[q] = a.acc_phiG[g];
+        rau[q] = a.acc_u[g];
+        rav[q] = a.acc_v[g];
       }
     }
-    {  // y sweep
-      const int mb = oy / H, pos = oy - mb * H;
-      const int r0 = mb * NX + ox;
-      if (pos == 0) {
-        fy = ryF[r0]; dvy = ryD[r0]; uy = ryU[r0];
-      } else {
-        const int b0 = (H + mb * H) * TX + lx;
-        double fe[DOF], fd[DOF], fu[DOF];
 #pragma unroll
-        for (int k = 0; k < DOF; ++k) {
-          const int l = b0 + k * TX;
-          fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
+    for (int q = 0; q < kP2; ++q) {
+      if (!live[q]) continue;
+      const int ox = rox[q], oy = roy[q];
+      const int lx = ox + H, ly = oy + H, li = ly * TX + lx;
+      double fx, dvx, vx, fy, dvy, uy;
+      {  // x sweep
+        const int mb = ox / H, pos = ox - mb * H;
+        const int r0 = mb * NY + oy;
+        if (pos == 0) {
+          fx = rxF[r0]; dvx = rxD[r0]; vx = rxV[r0];
+        } else {
+          const int b0 = ly * TX + H + mb * H;
+          double fe[DOF], fd[DOF], fv[DOF];
+#pragma unroll
+          for (int k = 0; k < DOF; ++k) {
+            fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
+          }
+          const int r1 = r0 + NY;
+          fx = cell_interior<DOF>(fe, rxF[r0], rxF[r1], pos, dx, dv);
+          dvx = cell_interior<DOF>(fd, rxD[r0], rxD[r1], pos, dx, dv);
+          vx = cell_interior<DOF>(fv, rxV[r0], rxV[r1], pos, dx, dv);
         }
-        const int r1 = r0 + NX;
-        fy = cell_interior<DOF>(fe, ryF[r0], ryF[r1], pos, dx, dv);
-        dvy = cell_interior<DOF>(fd, ryD[r0], ryD[r1], pos, dx, dv);
-        uy = cell_interior<DOF>(fu, ryU[r0], ryU[r1], pos, dx, dv);
       }
-    }
-    const int i = i0 + ox, j = j0 + oy;
-    const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
-    const long long g = pbase + off;
-    const Recip sgr{m.sqrtG[mbase + off], m.rsqrtG[mbase + off]};
-    const double sg = sgr.b;
-    const double fc = m.f[pbase + off];
-    const double cu = sCU[li], cv = sCV[li];
-    // spatial_operators_mod.F90:127-131
-    const double vort = div_by(vx - uy, sgr);
-    const double kpg = -dvx - dvy;
-    const double ku = -fx + sg * (vort + fc) * cv;
-    const double kv = -fy - sg * (vort + fc) * cu;
+      {  // y sweep
+        const int mb = oy / H, pos = oy - mb * H;
+        const int r0 = mb * NX + ox;
+        if (pos == 0) {
+          fy = ryF[r0]; dvy = ryD[r0]; uy = ryU[r0];
+        } else {
+          const int b0 = (H + mb * H) * TX + lx;
+          double fe[DOF], fd[DOF], fu[DOF];
+#pragma unroll
+          for (int k = 0; k < DOF; ++k) {
+            const int l = b0 + k * TX;
+            fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
+          }
+          const int r1 = r0 + NX;
+          fy = cell_interior<DOF>(fe, ryF[r0], ryF[r1], pos, dx, dv);
+          dvy = cell_interior<DOF>(fd, ryD[r0], ryD[r1], pos, dx, dv);
+          uy = cell_interior<DOF>(fu, ryU[r0], ryU[r1], pos, dx, dv);
+        }
+      }
+      const int i = i0 + ox, j = j0 + oy;
+      const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
+      const long long g = pbase + off;
+      const Recip sgr{rsg[q], rrs[q]};
+      const double sg = sgr.b;
+      const double fc = rf[q];
+      const double cu = sCU[li], cv = sCV[li];
+      // spatial_operators_mod.F90:127-131
+      const double vort = div_by(vx - uy, sgr);
+      const double kpg = -dvx - dvy;
+      const double ku = -fx + sg * (vort + fc) * cv;
+      const double kv = -fy - sg * (vort + fc) * cu;
 
-    if (kind == kTend) {
-      const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
-      a.o_phi[ti] = kpg;
-      a.o_u[ti] = ku;
-      a.o_v[ti] = kv;
-    } else {
-      double pgn, un, vn;  // base state Y_n
-      if (kind == kRK4S1 || kind == kRK3S1) {
-        pgn = sPG[li]; un = sU[li]; vn = sV[li];
+      if (kind == kTend) {
+        const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
+        a.o_phi[ti] = kpg;
+        a.o_u[ti] = ku;
+        a.o_v[ti] = kv;
       } else {
-        pgn = a.b_phi[g] * sg; un = a.b_u[g]; vn = a.b_v[g];
-      }
-      double pg, u, v;
-      if (kind == kRK4S1) {
-        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-        a.acc_phiG[g] = kpg; a.acc_u[g] = ku; a.acc_v[g] = kv;
-      } else if (kind == kRK4S2 || kind == kRK4S3) {
-        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-        a.acc_phiG[g] = a.acc_phiG[g] + 2.0 * kpg;
-        a.acc_u[g] = a.acc_u[g] + 2.0 * ku;
-        a.acc_v[g] = a.acc_v[g] + 2.0 * kv;
-      } else if (kind == kRK4S4) {
-        constexpr Recip k6{6.0, 1.0 / 6.0};
-        const double tpg = div_by(a.acc_phiG[g] + kpg, k6);
-        const double tu = div_by(a.acc_u[g] + ku, k6);
-        const double tv = div_by(a.acc_v[g] + kv, k6);
-        pg = pgn + a.c1 * tpg; u = un + a.c1 * tu; v = vn + a.c1 * tv;
-      } else if (kind == kRK3S1) {
-        pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-      } else if (kind == kRK3S2) {
-        pg = 0.75 * pgn + 0.25 * sPG[li] + a.c1 * kpg;
-        u = 0.75 * un + 0.25 * sU[li] + a.c1 * ku;
-        v = 0.75 * vn + 0.25 * sV[li] + a.c1 * kv;
-      } else {  // kRK3S3
-        constexpr Recip k3{3.0, 1.0 / 3.0};
-        pg = div_by(pgn, k3) + 2.0 / 3.0 * sPG[li] + a.c1 * kpg;
-        u = div_by(un, k3) + 2.0 / 3.0 * sU[li] + a.c1 * ku;
-        v = div_by(vn, k3) + 2.0 / 3.0 * sV[li] + a.c1 * kv;
-      }
-      const double phi = div_by(pg, sgr);  // time_scheme_mod.F90:81
-      a.o_phi[g] = phi;
-      a.o_u[g] = u;
-      a.o_v[g] = v;
-      // raw copy of edge-line points for the boundary pass
-      if (i == 1 || i == npv || j == 1 || j == npv) {
-        const double vals[3] = {phi, u, v};
+        double pgn, un, vn;  // base state Y_n
+        if (kind == kRK4S1 || kind == kRK3S1) {
+          pgn = sPG[li]; un = sU[li]; vn = sV[li];
+        } else {
+          pgn = rbp[q] * sg; un = rbu[q]; vn = rbv[q];
+        }
+        double pg, u, v;
+        if (kind == kRK4S1) {
+          pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
+          a.acc_phiG[g] = kpg; a.acc_u[g] = ku; a.acc_v[g] = kv;
+        } else if (kind == kRK4S2 || kind == kRK4S3) {
+          pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
+          a.acc_phiG[g] = rap[q] + 2.0 * kpg;
+          a.acc_u[g] = rau[q] + 2.0 * ku;
+          a.acc_v[g] = rav[q] + 2.0 * kv;
+        } else if (kind == kRK4S4) {
+          constexpr Recip k6{6.0, 1.0 / 6.0};
+          const double tpg = div_by(rap[q] + kpg, k6);
+          const double tu = div_by(rau[q] + ku, k6);
+          const double tv = div_by(rav[q] + kv, k6);
+          pg = pgn + a.c1 * tpg; u = un + a.c1 * tu; v = vn + a.c1 * tv;
+        } else if (kind == kRK3S1) {
+          pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
+        } else if (kind == kRK3S2) {
+          pg = 0.75 * pgn + 0.25 * sPG[li] + a.c1 * kpg;
+          u = 0.75 * un + 0.25 * sU[li] + a.c1 * ku;
+          v = 0.75 * vn + 0.25 * sV[li] + a.c1 * kv;
+        } else {  // kRK3S3
+          constexpr Recip k3{3.0, 1.0 / 3.0};
+          pg = div_by(pgn, k3) + 2.0 / 3.0 * sPG[li] + a.c1 * kpg;
+          u = div_by(un, k3) + 2.0 / 3.0 * sU[li] + a.c1 * ku;
+          v = div_by(vn, k3) + 2.0 / 3.0 * sV[li] + a.c1 * kv;
+        }
+        const double phi = div_by(pg, sgr);  // time_scheme_mod.F90:81
+        a.o_phi[g] = phi;
+        a.o_u[g] = u;
+        a.o_v[g] = v;
+        // raw copy of edge-line points for the boundary pass
+        if (i == 1 || i == npv || j == 1 || j == npv) {
+          const double vals[3] = {phi, u, v};
 #pragma unroll
-        for (int fl = 0; fl < 3; ++fl) {
-          if (i == 1) a.edge_raw[edge_slot(npv, p, 0, fl, j)] = vals[fl];
-          if (i == npv) a.edge_raw[edge_slot(npv, p, 1, fl, j)] = vals[fl];
-          if (j == 1) a.edge_raw[edge_slot(npv, p, 2, fl, i)] = vals[fl];
-          if (j == npv) a.edge_raw[edge_slot(npv, p, 3, fl, i)] = vals[fl];
+          for (int fl = 0; fl < 3; ++fl) {
+            if (i == 1) a.edge_raw[edge_slot(npv, p, 0, fl, j)] = vals[fl];
+            if (i == npv) a.edge_raw[edge_slot(npv, p, 1, fl, j)] = vals[fl];
+            if (j == 1) a.edge_raw[edge_slot(npv, p, 2, fl, i)] = vals[fl];
+            if (j == npv) a.edge_raw[edge_slot(npv, p, 3, fl, i)] = vals[fl];
+          }
         }
       }
     }

@@ -10,7 +10,8 @@ Requests (one seeded cubed-sphere solve each) arrive over time.  The loop:
     fused stage launch + one fused boundary launch per stage), each solve at
     its own stage of its own scheme;
   * retires finished solves (read-back, FP64 mass/energy) and frees their
-    slot and memory for the next admission.
+    slot and memory for the next admission; a solve whose state is no longer
+    finite is retired as failed ("non-finite") and reported.
 Latency is arrival to completed read-back.  The reference has no serving
 mode (one solve per process, MCV_SW.F90:48-77); the per-solve step is the
 reference's, bit for bit in FP64.
@@ -162,17 +163,26 @@ class Server:
                 busy += time.perf_counter() - tb
                 for slot in [s for s in active if left[s] == 0]:
                     r, need, t_admit = active.pop(slot)
-                    out = b.sync(slot, fields=keep)
+                    status = "ok"
+                    try:
+                        out = b.sync(slot, fields=keep)
+                    except capi.MCVError as e:
+                        if e.code != capi.MCV_ENONFINITE:
+                            raise
+                        status, out = "non-finite", None  # the solve blew up
                     b.remove(slot)
                     used -= need
                     t_done = time.perf_counter() - t0
                     lat.append(t_done - r.t_arrival)
-                    results[r.rid] = out
+                    if out is not None:
+                        results[r.rid] = out
                     rows.append({"rid": r.rid, "dof": r.dof, "n": r.n, "precision": r.precision,
+                                 "hours": r.hours, "u0": r.u0, "rms_h": r.rms_h,
                                  "steps": r.nsteps, "arrival": r.t_arrival, "admit": t_admit,
-                                 "done": t_done, "latency": t_done - r.t_arrival})
+                                 "done": t_done, "latency": t_done - r.t_arrival,
+                                 "status": status})
                     if log:
-                        log(f"request {r.rid} done: MCV{r.dof} n={r.n} "
+                        log(f"request {r.rid} {status}: MCV{r.dof} n={r.n} "
                             f"{'FP32C' if r.precision else 'FP64'} {r.nsteps} steps, "
                             f"latency {t_done - r.t_arrival:.2f} s")
         finally:
@@ -184,6 +194,7 @@ class Server:
         report = {
             "requests": len(reqs),
             "completed": int(len(lat)),
+            "non_finite": [x["rid"] for x in rows if x["status"] != "ok"],
             "makespan_s": span,
             "solves_per_s": len(lat) / span if span > 0 else 0.0,
             "cell_steps_per_s": cell_steps / span if span > 0 else 0.0,

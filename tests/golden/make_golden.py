@@ -8,7 +8,9 @@ compiled unmodified; see oracle/Makefile) in this container and records:
                 integrals of diag_mod.F90 at the first and last step, and
                 the reference's mesh digests
   tiny_*.npz    full step-0 and final arrays of the 4x4-cell cases
-Run: python tests/golden/make_golden.py   (needs `make -C oracle ref`)
+  *_full_ghost.npz  the reference's ghost tables (coef, iref) of the
+                full-size case
+Run: python tests/golden/make_golden.py [case ...]   (needs `make -C oracle ref`)
 """
 import hashlib
 import json
@@ -60,18 +62,32 @@ CASES = [
      "RK4", 240.0, 360, {}),
     ("mcv4_n16_jet_rk4", 4, 16, ("jet", dict(u0=40.0, h0=8000.0, lmax=16, seed=0, rms_h=60.0)),
      "RK4", 300.0, 24, {}),
+    # the single large solve at its full size (a few steps); its ghost tables
+    # are stored too (the reference's host passes its own tables)
+    ("mcv4_n384_jet_rk4_full", 4, 384,
+     ("jet", dict(u0=40.0, h0=8000.0, lmax=16, seed=0, rms_h=60.0)), "RK4", 10.0, 3, {}),
 ]
 
 MESHES = [(3, 4), (4, 4), (3, 8), (3, 32), (4, 12), (4, 16), (4, 48), (3, 90)]
 
 
-def main():
-    out = {"cases": {}, "meshes": {}}
-    for dof, n in MESHES:
+def main(only=None):
+    """Regenerate every case, or with `only` (case names) just those, keeping
+    the other records of golden.json."""
+    path = os.path.join(HERE, "golden.json")
+    if only:
+        with open(path) as fh:
+            out = json.load(fh)
+        run_cases = [c for c in CASES if c[0] in only]
+        meshes = []
+    else:
+        out = {"cases": {}, "meshes": {}}
+        run_cases, meshes = CASES, MESHES
+    for dof, n in meshes:
         r = ref_runner.run(dof, n, dt=60.0, nsteps=0, case_num=2)
         out["meshes"][f"{dof}_{n}"] = {k: digest(r["mesh"][k]) for k in MESH_KEYS}
         print("mesh", dof, n, flush=True)
-    for name, dof, n, init, scheme, dt, steps, extra in CASES:
+    for name, dof, n, init, scheme, dt, steps, extra in run_cases:
         ic = None
         case_num = 2
         if init[0] == "case":
@@ -90,7 +106,12 @@ def main():
                "steps": steps, "extra": extra,
                "step0": state_digests(s0, dof, n), "final": state_digests(s1, dof, n),
                "mass0": s0["mass"], "energy0": s0["energy"], "mass": s1["mass"],
-               "energy": s1["energy"], "phi_s": digest(r["mesh"]["phi_s"])}
+               "energy": s1["energy"]}
+        if "mesh" in r:
+            rec["phi_s"] = digest(r["mesh"]["phi_s"])
+        if name.endswith("_full"):
+            np.savez_compressed(os.path.join(HERE, name + "_ghost.npz"),
+                                coef=r["mesh"]["coef"], iref=r["mesh"]["iref"])
         out["cases"][name] = rec
         if name.startswith("tiny_"):
             arrs = {}
@@ -101,9 +122,9 @@ def main():
             arrs["phi_s"] = r["mesh"]["phi_s"]
             np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrs)
         print("case", name, flush=True)
-    with open(os.path.join(HERE, "golden.json"), "w") as fh:
+    with open(path, "w") as fh:
         json.dump(out, fh, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main()
+    main(only=sys.argv[1:] or None)
