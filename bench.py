@@ -254,6 +254,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--ensemble-stages", type=int, default=180,
+                    help="timed RK stages of the ensemble (a multiple of 3)")
     ap.add_argument("--serve-seconds", type=float, default=1.0,
                     help="seconds of Poisson arrivals for the serving run (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -332,7 +334,8 @@ def main():
         del out32
     del out64
     if not args.no_batch:
-        res["batched"] = run_ensemble(capi, hostlib, stages=3 * 60, warmup_stages=3 * 4)
+        res["batched"] = run_ensemble(capi, hostlib, stages=max(3, args.ensemble_stages // 3 * 3),
+                                      warmup_stages=3 * 4)
         log(f"ensemble: {res['batched']['value']:.3e} cell-steps/s")
     if args.serve_seconds > 0:
         res["serve"] = run_serve(capi, args.serve_seconds)
