@@ -20,6 +20,8 @@ Also reported on the same line:
   roofline     the FP64 stage kernel's achieved algorithmic HBM bandwidth
                (51 FP64 words per point per RK4 step, 9 points per MCV4 cell)
                from HIP events on the solver's stream, against 8 TB/s
+  batched_uneven  128 solves of mixed order and size (n in 32..256) in one
+               batch, FP32C, RK4, CFL time steps; cell-steps/s of all solves
   serve        the serving loop: Poisson arrivals (20 req/s) of seeded solves
                of mixed order, size, length and precision, continuous batching
                with admission by HBM footprint; completed solves/s and p50/p99
@@ -205,6 +207,62 @@ def run_ensemble(capi, hostlib, stages, warmup_stages):
             "members_done_steps": int(done[:E_MEMBERS].min())}
 
 
+def run_uneven(capi, hostlib, count=128, stages=24, seed=0):
+    """Batched uneven sizes: `count` solves in one batch, MCV3/MCV4 50/50, panel
+    size uniform in {32, 48, 90, 128, 180, 256}, FP32C, RK4 with dt at Courant
+    0.5 on the fastest gravity wave, balanced jets with u0 ~ U(10, 40) m/s and
+    perturbation rms ~ U(10, 50) m; autotuned; cell-steps/s over all solves."""
+    import concurrent.futures as cf
+    from mcv_sw_amd import serve
+
+    rng = np.random.default_rng(seed)
+    sizes = (32, 48, 90, 128, 180, 256)
+    specs = [dict(dof=4 if rng.random() < 0.5 else 3, n=int(rng.choice(sizes)),
+                  u0=float(rng.uniform(10.0, 40.0)), rms=float(rng.uniform(10.0, 50.0)),
+                  seed=2000 + k) for k in range(count)]
+    t0 = time.time()
+    meshes = {}
+    for key in sorted({(sp["dof"], sp["n"]) for sp in specs}):
+        meshes[key] = hostlib.mesh(*key)
+
+    def prep(sp):
+        m = dict(meshes[(sp["dof"], sp["n"])])
+        return m, hostlib.jet(m, u0=sp["u0"], h0=5960.0, lmax=8, seed=sp["seed"],
+                              rms_h=sp["rms"])
+
+    with cf.ThreadPoolExecutor(16) as pool:
+        states = list(pool.map(prep, specs))
+    log(f"uneven batch: {count} initial states in {time.time() - t0:.1f} s")
+    steps = stages // 4
+    cells = 0
+    with capi.Batch(count) as b:
+        for sp, (m, st) in zip(specs, states):
+            dt = serve.cfl_dt(sp["dof"], sp["n"], sp["u0"], 5960.0)
+            b.add(sp["dof"], sp["n"], dt, m, st, 1000, scheme="RK4", precision=capi.FP32C)
+            cells += 6 * sp["n"] ** 2
+        del states
+        tile, tile_ms = b.autotune(reps=3)
+        b.run(8)
+        b.poll()
+        t = time.perf_counter()
+        b.run(stages)
+        b.poll()
+        wall = time.perf_counter() - t
+        stage_ms, bnd_ms = b.run_timed(stages)
+    mix = {}
+    for sp in specs:
+        k = f"mcv{sp['dof']}_n{sp['n']}"
+        mix[k] = mix.get(k, 0) + 1
+    return {"workload": f"{count} solves in one batch: MCV3/MCV4 50/50, n uniform in "
+                        f"{{32,48,90,128,180,256}}, FP32C, RK4, CFL dt (Courant 0.5), one fused "
+                        f"launch per stage, autotuned tile shapes",
+            "value": cells * steps / wall, "unit": "cell-steps/s", "cells": cells,
+            "ms_per_step": wall * 1e3 / steps, "stages_timed": stages,
+            "stage_kernel_ms_per_step": stage_ms / steps,
+            "boundary_kernel_ms_per_step": bnd_ms / steps,
+            "tile_variant": tile, "tile_variant_ms_per_stage": tile_ms, "mix": mix}
+
+
 def run_serve(capi, seconds, rate=20.0, seed=0, check=6):
     """The serving workload: Poisson arrivals at `rate` req/s for `seconds`,
     one seeded solve per request (order, size, length and precision drawn as
@@ -337,6 +395,9 @@ def main():
         res["batched"] = run_ensemble(capi, hostlib, stages=max(3, args.ensemble_stages // 3 * 3),
                                       warmup_stages=3 * 4)
         log(f"ensemble: {res['batched']['value']:.3e} cell-steps/s")
+    if not args.no_batch:
+        res["batched_uneven"] = run_uneven(capi, hostlib)
+        log(f"uneven batch: {res['batched_uneven']['value']:.3e} cell-steps/s")
     if args.serve_seconds > 0:
         res["serve"] = run_serve(capi, args.serve_seconds)
         log(f"serve: {res['serve']['completed']} solves, p50 {res['serve']['latency_p50_s']:.2f} s, "

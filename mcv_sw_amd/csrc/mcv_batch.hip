@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
     if (S.prec == MCV_FP64) {
       StageArgs a = S.a64[ph];
       if (st.first) { a.cu0 = S.cu0; a.cv0 = S.cv0; }
-      const MeshDev m = S.mesh;
+      const MeshDev &m = S.mesh;  // read where used (scalar loads), not copied
       if (S.dof == 3) {
         const int tx = tiles_x_of<V>(3, MCV_FP64, n);
         stage_tile<3, TileSet<3>::cx[V], TileSet<3>::cy[V]>(m, a, kind, p, tile % tx, tile / tx,
@@ -146,8 +146,8 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
                                                              smem);
       }
     } else {
-      const StageArgs32 a = S.a32[ph];
-      const MeshF32 m = S.mesh32;
+      const StageArgs32 &a = S.a32[ph];
+      const MeshF32 &m = S.mesh32;
       if (S.dof == 3) {
         const int tx = tiles_x_of<V>(3, MCV_FP32C, n);
         stage_tile_f32<3, TileSet32<3>::cx[V], TileSet32<3>::cy[V]>(m, a, kind, p, tile % tx,

@@ -112,6 +112,25 @@ __host__ __device__ inline void edge_ij(int npv, int side, int s, int &i, int &j
   else { i = s; j = npv; }
 }
 
+// Element idx of a device array addressed as a uniform base plus a 32-bit
+// byte offset, which the global load/store instructions take directly (no
+// 64-bit address arithmetic per access).  The library keeps every array that
+// the stage kernels touch below 2^32 bytes (n <= 2048).
+template <typename T>
+__device__ __forceinline__ T ldg(const T *base, uint32_t idx) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      (size_t)(idx * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stg(T *base, uint32_t idx, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (size_t)(idx * (uint32_t)sizeof(T))) = v;
+}
+
+// 32-bit slot of the raw edge-line copy (see edge_slot).
+__host__ __device__ inline uint32_t edge_slot32(int npv, int p, int side, int field, int s) {
+  return ((uint32_t)((p - 1) * 4 + side) * 3u + (uint32_t)field) * (uint32_t)npv + (uint32_t)(s - 1);
+}
+
 // Linear index of point (i, j) of panel p (1-based, halo-relative i, j).
 __host__ __device__ inline long long pidx(int nh, int H, int p, int i, int j) {
   return ((long long)(p - 1) * nh + (j + H - 1)) * (long long)nh + (i + H - 1);

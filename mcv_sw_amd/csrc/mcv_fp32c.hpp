@@ -108,8 +108,9 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   const int TXo = ncx * H + (cx0 + ncx == N ? 1 : 0);
   const int TYo = ncy * H + (cy0 + ncy == N ? 1 : 0);
   const int lw = nx + 2 * H, lh = ny + 2 * H;
-  const long long pbase = (long long)(p - 1) * nh * nh;
-  const long long mbase = (long long)(p - 1) * m.metric_pstride;
+  // 32-bit point indices (every array here is below 2^32 bytes)
+  const uint32_t pbase = (uint32_t)(p - 1) * (uint32_t)(nh * nh);
+  const uint32_t mbase = (uint32_t)(p - 1) * (uint32_t)m.metric_pstride;
 
   // loads of all a thread's points first, then the derived values
   constexpr int kP0 = (TX * TY + kStageThreads - 1) / kStageThreads;
@@ -124,17 +125,17 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
       live[q] = t < TX * TY && lx < lw && ly < lh && (inx || iny);
       if (!live[q]) continue;
-      const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
-      const long long g = pbase + off;
-      rdp[q] = a.in_phi[g];
-      ru[q] = a.in_u[g];
-      rv[q] = a.in_v[g];
-      rpg[q] = m.pgref[g];
-      r11[q] = m.ig11[mbase + off];
-      r12[q] = m.ig12[mbase + off];
-      r22[q] = m.ig22[mbase + off];
-      rsg[q] = m.sg[mbase + off];
-      rrs[q] = m.rsg[mbase + off];
+      const uint32_t off = (uint32_t)((j0 - 1 + ly) * nh + (i0 - 1 + lx));
+      const uint32_t g = pbase + off;
+      rdp[q] = ldg(a.in_phi, g);
+      ru[q] = ldg(a.in_u, g);
+      rv[q] = ldg(a.in_v, g);
+      rpg[q] = ldg(m.pgref, g);
+      r11[q] = ldg(m.ig11, mbase + off);
+      r12[q] = ldg(m.ig12, mbase + off);
+      r22[q] = ldg(m.ig22, mbase + off);
+      rsg[q] = ldg(m.sg, mbase + off);
+      rrs[q] = ldg(m.rsg, mbase + off);
     }
 #pragma unroll
     for (int q = 0; q < kP0; ++q) {
@@ -252,29 +253,29 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       rox[q] = ox;
       roy[q] = oy;
       if (!ok) continue;
-      const long long off = (long long)(j0 + oy + H - 1) * nh + (i0 + ox + H - 1);
-      const long long g = pbase + off;
-      rsg[q] = m.sg[mbase + off];
-      rrs[q] = m.rsg[mbase + off];
-      rf[q] = m.f[g];
-      rdx_[q] = m.dEx[g];
-      rdy_[q] = m.dEy[g];
+      const uint32_t off = (uint32_t)((j0 + oy + H - 1) * nh + (i0 + ox + H - 1));
+      const uint32_t g = pbase + off;
+      rsg[q] = ldg(m.sg, mbase + off);
+      rrs[q] = ldg(m.rsg, mbase + off);
+      rf[q] = ldg(m.f, g);
+      rdx_[q] = ldg(m.dEx, g);
+      rdy_[q] = ldg(m.dEy, g);
       if (kind == kTend) continue;
       if (last) {
-        rcp[q] = a.c_phi[g];
-        rcu[q] = a.c_u[g];
-        rcv[q] = a.c_v[g];
+        rcp[q] = ldg(a.c_phi, g);
+        rcu[q] = ldg(a.c_u, g);
+        rcv[q] = ldg(a.c_v, g);
       }
-      if (need_in) rin[q] = a.in_phi[g];
+      if (need_in) rin[q] = ldg(a.in_phi, g);
       if (!first) {
-        rbp[q] = a.b_phi[g];
-        rbu[q] = a.b_u[g];
-        rbv[q] = a.b_v[g];
+        rbp[q] = ldg(a.b_phi, g);
+        rbu[q] = ldg(a.b_u, g);
+        rbv[q] = ldg(a.b_v, g);
       }
       if (need_acc) {
-        rap[q] = a.acc_phiG[g];
-        rau[q] = a.acc_u[g];
-        rav[q] = a.acc_v[g];
+        rap[q] = ldg(a.acc_phiG, g);
+        rau[q] = ldg(a.acc_u, g);
+        rav[q] = ldg(a.acc_v, g);
       }
     }
 #pragma unroll
@@ -321,8 +322,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         }
       }
       const int i = i0 + ox, j = j0 + oy;
-      const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
-      const long long g = pbase + off;
+      const uint32_t g = pbase + (uint32_t)((j + H - 1) * nh + (i + H - 1));
       // tendencies assembled in FP64 (spatial_operators_mod.F90:127-131)
       const double sg = rsg[q], rsgv = rrs[q];
       const double fc = (double)rf[q];
@@ -333,10 +333,10 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       const double kv = -((double)rdy_[q] + (double)fy) - sg * (vort + fc) * cu;
       const double kph = kpg * rsgv;  // d(phi')/dt = d(phiG)/dt / sqrtG
       if (kind == kTend) {
-        const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
-        a.o_phi[ti] = (float)kpg;
-        a.o_u[ti] = (float)ku;
-        a.o_v[ti] = (float)kv;
+        const uint32_t ti = ((uint32_t)(p - 1) * npv + (j - 1)) * npv + (i - 1);
+        stg(a.o_phi, ti, (float)kpg);
+        stg(a.o_u, ti, (float)ku);
+        stg(a.o_v, ti, (float)kv);
       } else {
         // base state Y_n: hi (+ lo in the last stage) in FP64
         double bph, bu, bv;
@@ -357,12 +357,12 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         double ph, u, v;
         if (kind == kRK4S1) {
           ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-          a.acc_phiG[g] = (float)kph; a.acc_u[g] = (float)ku; a.acc_v[g] = (float)kv;
+          stg(a.acc_phiG, g, (float)kph); stg(a.acc_u, g, (float)ku); stg(a.acc_v, g, (float)kv);
         } else if (kind == kRK4S2 || kind == kRK4S3) {
           ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-          a.acc_phiG[g] = (float)((double)rap[q] + 2.0 * kph);
-          a.acc_u[g] = (float)((double)rau[q] + 2.0 * ku);
-          a.acc_v[g] = (float)((double)rav[q] + 2.0 * kv);
+          stg(a.acc_phiG, g, (float)((double)rap[q] + 2.0 * kph));
+          stg(a.acc_u, g, (float)((double)rau[q] + 2.0 * ku));
+          stg(a.acc_v, g, (float)((double)rav[q] + 2.0 * kv));
         } else if (kind == kRK4S4) {
           ph = bph + a.c1 * (((double)rap[q] + kph) * (1.0 / 6.0));
           u = bu + a.c1 * (((double)rau[q] + ku) * (1.0 / 6.0));
@@ -379,13 +379,13 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           v = bv + (2.0 / 3.0) * ((double)sV[li] - bv) + a.c1 * kv;
         }
         const float hph = (float)ph, hu = (float)u, hv = (float)v;
-        a.o_phi[g] = hph;
-        a.o_u[g] = hu;
-        a.o_v[g] = hv;
+        stg(a.o_phi, g, hph);
+        stg(a.o_u, g, hu);
+        stg(a.o_v, g, hv);
         if (a.oc_phi) {  // last stage: keep the rounding residue (Kahan word)
-          a.oc_phi[g] = (float)(ph - (double)hph);
-          a.oc_u[g] = (float)(u - (double)hu);
-          a.oc_v[g] = (float)(v - (double)hv);
+          stg(a.oc_phi, g, (float)(ph - (double)hph));
+          stg(a.oc_u, g, (float)(u - (double)hu));
+          stg(a.oc_v, g, (float)(v - (double)hv));
         }
         if (i == 1 || i == npv || j == 1 || j == npv) {
           const float vals[3] = {hph, hu, hv};
@@ -397,15 +397,15 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           }
 #pragma unroll
           for (int fl = 0; fl < 3; ++fl) {
-            long long sl[4];
+            uint32_t sl[4];
             int ns = 0;
-            if (i == 1) sl[ns++] = edge_slot(npv, p, 0, fl, j);
-            if (i == npv) sl[ns++] = edge_slot(npv, p, 1, fl, j);
-            if (j == 1) sl[ns++] = edge_slot(npv, p, 2, fl, i);
-            if (j == npv) sl[ns++] = edge_slot(npv, p, 3, fl, i);
+            if (i == 1) sl[ns++] = edge_slot32(npv, p, 0, fl, j);
+            if (i == npv) sl[ns++] = edge_slot32(npv, p, 1, fl, j);
+            if (j == 1) sl[ns++] = edge_slot32(npv, p, 2, fl, i);
+            if (j == npv) sl[ns++] = edge_slot32(npv, p, 3, fl, i);
             for (int e = 0; e < ns; ++e) {
-              a.edge_raw[sl[e]] = vals[fl];
-              if (a.oc_phi) a.edge_raw_c[sl[e]] = lows[fl];
+              stg(a.edge_raw, sl[e], vals[fl]);
+              if (a.oc_phi) stg(a.edge_raw_c, sl[e], lows[fl]);
             }
           }
         }

@@ -25,28 +25,29 @@ int fail(int code, const char *fmt, ...);
   } while (0)
 
 // Tile shapes (cells x cells per workgroup) the autotuner chooses from, FP64
-// and FP32C.
+// and FP32C.  Phase 2 of the stage kernel visits H*H*CX*CY interior slots plus
+// H*(CX+CY)+1 far-end points; 6x4 (MCV4: 247) and 8x7 (MCV3: 255) fill 256
+// threads in one pass, the larger shapes re-read less halo per cell.
 constexpr int kTileVariants = 4;
 template <int DOF> struct TileSet;
 template <> struct TileSet<3> {
-  static constexpr int cx[kTileVariants] = {8, 16, 8, 12};
-  static constexpr int cy[kTileVariants] = {8, 4, 4, 6};
+  static constexpr int cx[kTileVariants] = {8, 16, 10, 12};
+  static constexpr int cy[kTileVariants] = {7, 4, 5, 6};
 };
 template <> struct TileSet<4> {
-  static constexpr int cx[kTileVariants] = {6, 8, 4, 10};
-  static constexpr int cy[kTileVariants] = {6, 4, 4, 4};
+  static constexpr int cx[kTileVariants] = {6, 8, 5, 6};
+  static constexpr int cy[kTileVariants] = {4, 4, 5, 6};
 };
 
 template <int DOF> struct TileSet32;
 template <> struct TileSet32<3> {
-  static constexpr int cx[kTileVariants] = {8, 16, 16, 8};
-  static constexpr int cy[kTileVariants] = {8, 8, 4, 4};
+  static constexpr int cx[kTileVariants] = {8, 16, 10, 16};
+  static constexpr int cy[kTileVariants] = {7, 8, 5, 4};
 };
 template <> struct TileSet32<4> {
   static constexpr int cx[kTileVariants] = {6, 8, 8, 10};
-  static constexpr int cy[kTileVariants] = {6, 4, 8, 6};
+  static constexpr int cy[kTileVariants] = {4, 4, 8, 6};
 };
-
 
 // Upper bound of the device bytes one solve of this configuration allocates
 // (exact when the metric terms differ between panels).

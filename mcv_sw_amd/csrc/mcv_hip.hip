@@ -140,7 +140,8 @@ int validate(const mcv_config *c) {
   if (c->abi_version != MCV_ABI_VERSION)
     return fail(MCV_EINVAL, "abi_version %d, library is %d", c->abi_version, MCV_ABI_VERSION);
   if (c->dof != 3 && c->dof != 4) return fail(MCV_EINVAL, "dof must be 3 or 4, got %d", c->dof);
-  if (c->n < 2 || c->n > 8192) return fail(MCV_EINVAL, "n must be in [2, 8192], got %d", c->n);
+  // every per-point array the stage kernels index stays below 2^32 bytes
+  if (c->n < 2 || c->n > 2048) return fail(MCV_EINVAL, "n must be in [2, 2048], got %d", c->n);
   if (c->scheme != MCV_RK4 && c->scheme != MCV_RK3_TVD)
     return fail(MCV_EINVAL, "Unknown integral scheme %d, select MCV_RK3_TVD or MCV_RK4",
                 c->scheme);

@@ -68,9 +68,10 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
   const int TXo = ncx * H + (cx0 + ncx == N ? 1 : 0);     // owned points
   const int TYo = ncy * H + (cy0 + ncy == N ? 1 : 0);
   const int lw = nx + 2 * H, lh = ny + 2 * H;
-  const long long pbase = (long long)(p - 1) * nh * nh;
-  const long long mbase = (long long)(p - 1) * m.metric_pstride;
-  const long long sbase = (long long)(p - 1) * m.phis_pstride;
+  // 32-bit point indices (every array here is below 2^32 bytes)
+  const uint32_t pbase = (uint32_t)(p - 1) * (uint32_t)(nh * nh);
+  const uint32_t mbase = (uint32_t)(p - 1) * (uint32_t)m.metric_pstride;
+  const uint32_t sbase = (uint32_t)(p - 1) * (uint32_t)m.phis_pstride;
 
   // Phase 0: stage the cross-shaped tile + halo; derived fields per point.
   // Every thread first issues the loads of all its points, then derives and
@@ -88,19 +89,19 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       // outside the tile, or a halo corner (never read)
       live[q] = t < TX * TY && lx < lw && ly < lh && (inx || iny);
       if (!live[q]) continue;
-      const long long off = (long long)(j0 - 1 + ly) * nh + (i0 - 1 + lx);
-      const long long g = pbase + off;
-      rphi[q] = a.in_phi[g];
-      ru[q] = a.in_u[g];
-      rv[q] = a.in_v[g];
-      rsg[q] = m.sqrtG[mbase + off];
-      r11[q] = m.ig11[mbase + off];
-      r12[q] = m.ig12[mbase + off];
-      r22[q] = m.ig22[mbase + off];
-      rps[q] = m.phi_s[sbase + off];
+      const uint32_t off = (uint32_t)((j0 - 1 + ly) * nh + (i0 - 1 + lx));
+      const uint32_t g = pbase + off;
+      rphi[q] = ldg(a.in_phi, g);
+      ru[q] = ldg(a.in_u, g);
+      rv[q] = ldg(a.in_v, g);
+      rsg[q] = ldg(m.sqrtG, mbase + off);
+      r11[q] = ldg(m.ig11, mbase + off);
+      r12[q] = ldg(m.ig12, mbase + off);
+      r22[q] = ldg(m.ig22, mbase + off);
+      rps[q] = ldg(m.phi_s, sbase + off);
       if (a.cu0) {
-        rcu[q] = a.cu0[g];
-        rcv[q] = a.cv0[g];
+        rcu[q] = ldg(a.cu0, g);
+        rcv[q] = ldg(a.cv0, g);
       }
     }
 #pragma unroll
@@ -217,20 +218,20 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       rox[q] = ox;
       roy[q] = oy;
       if (!ok) continue;
-      const long long off = (long long)(j0 + oy + H - 1) * nh + (i0 + ox + H - 1);
-      const long long g = pbase + off;
-      rsg[q] = m.sqrtG[mbase + off];
-      rrs[q] = m.rsqrtG[mbase + off];
-      rf[q] = m.f[g];
+      const uint32_t off = (uint32_t)((j0 + oy + H - 1) * nh + (i0 + ox + H - 1));
+      const uint32_t g = pbase + off;
+      rsg[q] = ldg(m.sqrtG, mbase + off);
+      rrs[q] = ldg(m.rsqrtG, mbase + off);
+      rf[q] = ldg(m.f, g);
       if (need_base) {
-        rbp[q] = a.b_phi[g];
-        rbu[q] = a.b_u[g];
-        rbv[q] = a.b_v[g];
+        rbp[q] = ldg(a.b_phi, g);
+        rbu[q] = ldg(a.b_u, g);
+        rbv[q] = ldg(a.b_v, g);
       }
       if (need_acc) {
-        rap[q] = a.acc_phiG[g];
-        rau[q] = a.acc_u[g];
-        rav[q] = a.acc_v[g];
+        rap[q] = ldg(a.acc_phiG, g);
+        rau[q] = ldg(a.acc_u, g);
+        rav[q] = ldg(a.acc_v, g);
       }
     }
 #pragma unroll
@@ -277,8 +278,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
         }
       }
       const int i = i0 + ox, j = j0 + oy;
-      const long long off = (long long)(j + H - 1) * nh + (i + H - 1);
-      const long long g = pbase + off;
+      const uint32_t g = pbase + (uint32_t)((j + H - 1) * nh + (i + H - 1));
       const Recip sgr{rsg[q], rrs[q]};
       const double sg = sgr.b;
       const double fc = rf[q];
@@ -290,10 +290,10 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       const double kv = -fy - sg * (vort + fc) * cu;
 
       if (kind == kTend) {
-        const long long ti = ((long long)(p - 1) * npv + (j - 1)) * npv + (i - 1);
-        a.o_phi[ti] = kpg;
-        a.o_u[ti] = ku;
-        a.o_v[ti] = kv;
+        const uint32_t ti = ((uint32_t)(p - 1) * npv + (j - 1)) * npv + (i - 1);
+        stg(a.o_phi, ti, kpg);
+        stg(a.o_u, ti, ku);
+        stg(a.o_v, ti, kv);
       } else {
         double pgn, un, vn;  // base state Y_n
         if (kind == kRK4S1 || kind == kRK3S1) {
@@ -304,12 +304,12 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
         double pg, u, v;
         if (kind == kRK4S1) {
           pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-          a.acc_phiG[g] = kpg; a.acc_u[g] = ku; a.acc_v[g] = kv;
+          stg(a.acc_phiG, g, kpg); stg(a.acc_u, g, ku); stg(a.acc_v, g, kv);
         } else if (kind == kRK4S2 || kind == kRK4S3) {
           pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-          a.acc_phiG[g] = rap[q] + 2.0 * kpg;
-          a.acc_u[g] = rau[q] + 2.0 * ku;
-          a.acc_v[g] = rav[q] + 2.0 * kv;
+          stg(a.acc_phiG, g, rap[q] + 2.0 * kpg);
+          stg(a.acc_u, g, rau[q] + 2.0 * ku);
+          stg(a.acc_v, g, rav[q] + 2.0 * kv);
         } else if (kind == kRK4S4) {
           constexpr Recip k6{6.0, 1.0 / 6.0};
           const double tpg = div_by(rap[q] + kpg, k6);
@@ -329,18 +329,18 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
           v = div_by(vn, k3) + 2.0 / 3.0 * sV[li] + a.c1 * kv;
         }
         const double phi = div_by(pg, sgr);  // time_scheme_mod.F90:81
-        a.o_phi[g] = phi;
-        a.o_u[g] = u;
-        a.o_v[g] = v;
+        stg(a.o_phi, g, phi);
+        stg(a.o_u, g, u);
+        stg(a.o_v, g, v);
         // raw copy of edge-line points for the boundary pass
         if (i == 1 || i == npv || j == 1 || j == npv) {
           const double vals[3] = {phi, u, v};
 #pragma unroll
           for (int fl = 0; fl < 3; ++fl) {
-            if (i == 1) a.edge_raw[edge_slot(npv, p, 0, fl, j)] = vals[fl];
-            if (i == npv) a.edge_raw[edge_slot(npv, p, 1, fl, j)] = vals[fl];
-            if (j == 1) a.edge_raw[edge_slot(npv, p, 2, fl, i)] = vals[fl];
-            if (j == npv) a.edge_raw[edge_slot(npv, p, 3, fl, i)] = vals[fl];
+            if (i == 1) stg(a.edge_raw, edge_slot32(npv, p, 0, fl, j), vals[fl]);
+            if (i == npv) stg(a.edge_raw, edge_slot32(npv, p, 1, fl, j), vals[fl]);
+            if (j == 1) stg(a.edge_raw, edge_slot32(npv, p, 2, fl, i), vals[fl]);
+            if (j == npv) stg(a.edge_raw, edge_slot32(npv, p, 3, fl, i), vals[fl]);
           }
         }
       }
@@ -351,7 +351,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
 // Single-solve stage kernel: one workgroup per (panel, tile), blocks of one
 // XCD on neighbouring tiles.
 template <int DOF, int KIND, int CX, int CY>
-__global__ __launch_bounds__(kStageThreads) void stage_kernel(MeshDev m, StageArgs a,
+__global__ __launch_bounds__(kStageThreads, 5) void stage_kernel(MeshDev m, StageArgs a,
                                                               int tiles_x) {
   __shared__ __align__(16) char smem[TileLayout<double, DOF, CX, CY>::kBytes];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
