@@ -58,7 +58,10 @@ struct MeshDev {
   const double *f;           // per panel (f differs between panels 5 and 6)
   const double *phi_s;
   const double *p2sp;        // (A*IG) per point, 4 doubles (m11, m21, m12, m22)
-  const double *sp2p;        // (G*IA) per point, 4 doubles
+  // Line-major copies for the boundary pass (4 doubles per point):
+  const double *p2sp_band;   // (A*IG) on the band lines, see band_point()
+  const double *sp2p_ghost;  // (G*IA) at the ghost points, see ghost_point()
+  const double *sp2p_edge;   // (G*IA) on the edge lines, see band_point() with d = 1
   const double *gcoef;       // (H, nPV): ghost offsets (ghost_mod.f90:11-18)
   const int32_t *giref;      // (H, nPV): 1-based cells
 };
@@ -69,14 +72,14 @@ struct StageArgs {
   const double *b_phi, *b_u, *b_v;     // RK base state Y_n
   double *acc_phiG, *acc_u, *acc_v;    // RK4 tendency accumulator
   double *o_phi, *o_u, *o_v;           // stage output (interior, raw)
-  double *edge_raw;                    // raw copy of the 4 edge lines, see edge_slot()
+  double *band_raw;                    // raw copy of the lines nearest each side, see band_slot()
   const double *cu0, *cv0;             // host contravariant winds (first stage only)
   double c1;                           // stage increment coefficient (x dt)
 };
 
 struct BoundaryArgs {
   double *phi, *u, *v;                 // level to finish (halo + edges)
-  const double *edge_raw;
+  const double *band_raw;
 };
 
 // Stage kinds of time_scheme_mod.F90:13-123.
@@ -91,10 +94,40 @@ enum StageKind : int {
   kRK3S3 = 7,  // Yn+1 = Yn/3 + 2/3 Y2 + 2/3 dt k3
 };
 
-// Slot of the raw edge-line copy: panel p (1..6), side (0 W, 1 E, 2 S, 3 N),
-// field (0 phi, 1 u, 2 v), along-edge index s (1..nPV).
-__host__ __device__ inline long long edge_slot(int npv, int p, int side, int field, int s) {
-  return ((long long)((p - 1) * 4 + side) * 3 + field) * npv + (s - 1);
+// The band: the H+1 lines nearest each side (0 W, 1 E, 2 S, 3 N) of each
+// panel p (1..6), as the stage kernel wrote them (before the boundary pass).
+// Depth d = 1 is the edge line; the along-line index al (1..nPV) is j on the
+// W/E sides and i on the S/N sides.  The boundary pass reads only the band,
+// contiguously along the lines.
+__host__ __device__ inline uint32_t band_point(int npv, int H, int p, int side, int d, int al) {
+  return (((uint32_t)(p - 1) * 4u + (uint32_t)side) * (uint32_t)(H + 1) + (uint32_t)(d - 1)) *
+             (uint32_t)npv + (uint32_t)(al - 1);
+}
+// field 0 phi, 1 u, 2 v
+__host__ __device__ inline uint32_t band_slot(int npv, int H, int p, int side, int d, int field,
+                                              int al) {
+  return ((((uint32_t)(p - 1) * 4u + (uint32_t)side) * (uint32_t)(H + 1) + (uint32_t)(d - 1)) * 3u +
+          (uint32_t)field) * (uint32_t)npv + (uint32_t)(al - 1);
+}
+// Ghost point at depth k (1..H) beyond side `side` of panel p, along index s.
+__host__ __device__ inline uint32_t ghost_point(int npv, int H, int p, int side, int k, int s) {
+  return (((uint32_t)(p - 1) * 4u + (uint32_t)side) * (uint32_t)H + (uint32_t)(k - 1)) *
+             (uint32_t)npv + (uint32_t)(s - 1);
+}
+// (i, j) of band point (side, depth d, along index al).
+__host__ __device__ inline void band_ij(int npv, int side, int d, int al, int &i, int &j) {
+  if (side == 0) { i = d; j = al; }
+  else if (side == 1) { i = npv + 1 - d; j = al; }
+  else if (side == 2) { i = al; j = d; }
+  else { i = al; j = npv + 1 - d; }
+}
+// Edge-line point (depth 1) of side `side` of panel p.
+__host__ __device__ inline uint32_t edge_point(int npv, int p, int side, int al) {
+  return ((uint32_t)(p - 1) * 4u + (uint32_t)side) * (uint32_t)npv + (uint32_t)(al - 1);
+}
+// Band lines per solve (all panels, sides, depths and fields).
+__host__ __device__ inline size_t band_values(int dof, int npv) {
+  return (size_t)6 * 4 * dof * 3 * npv;
 }
 
 // Work items of the boundary pass on one level: every ghost point (6 panels
@@ -126,10 +159,6 @@ __device__ __forceinline__ void stg(T *base, uint32_t idx, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (size_t)(idx * (uint32_t)sizeof(T))) = v;
 }
 
-// 32-bit slot of the raw edge-line copy (see edge_slot).
-__host__ __device__ inline uint32_t edge_slot32(int npv, int p, int side, int field, int s) {
-  return ((uint32_t)((p - 1) * 4 + side) * 3u + (uint32_t)field) * (uint32_t)npv + (uint32_t)(s - 1);
-}
 
 // Linear index of point (i, j) of panel p (1-based, halo-relative i, j).
 __host__ __device__ inline long long pidx(int nh, int H, int p, int i, int j) {

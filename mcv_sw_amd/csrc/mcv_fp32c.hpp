@@ -41,8 +41,8 @@ struct StageArgs32 {
   float *acc_phiG, *acc_u, *acc_v;         // RK4 accumulator
   float *o_phi, *o_u, *o_v;                // stage output (raw interior)
   float *oc_phi, *oc_u, *oc_v;             // compensation of the output (last stage only)
-  float *edge_raw;                         // raw edge lines (hi)
-  float *edge_raw_c;                       // raw edge lines (lo), last stage only
+  float *band_raw;                         // raw band lines (hi), see band_slot()
+  float *band_raw_c;                       // raw band lines (lo), last stage only
   double c1;                               // stage increment coefficient (x dt)
 };
 
@@ -387,7 +387,8 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           stg(a.oc_u, g, (float)(u - (double)hu));
           stg(a.oc_v, g, (float)(v - (double)hv));
         }
-        if (i == 1 || i == npv || j == 1 || j == npv) {
+        // raw band lines (and their residues in the last stage) for the boundary pass
+        if (i <= H + 1 || i >= npv - H || j <= H + 1 || j >= npv - H) {
           const float vals[3] = {hph, hu, hv};
           float lows[3] = {0.0f, 0.0f, 0.0f};
           if (a.oc_phi) {
@@ -399,13 +400,13 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           for (int fl = 0; fl < 3; ++fl) {
             uint32_t sl[4];
             int ns = 0;
-            if (i == 1) sl[ns++] = edge_slot32(npv, p, 0, fl, j);
-            if (i == npv) sl[ns++] = edge_slot32(npv, p, 1, fl, j);
-            if (j == 1) sl[ns++] = edge_slot32(npv, p, 2, fl, i);
-            if (j == npv) sl[ns++] = edge_slot32(npv, p, 3, fl, i);
+            if (i <= H + 1) sl[ns++] = band_slot(npv, H, p, 0, i, fl, j);
+            if (i >= npv - H) sl[ns++] = band_slot(npv, H, p, 1, npv + 1 - i, fl, j);
+            if (j <= H + 1) sl[ns++] = band_slot(npv, H, p, 2, j, fl, i);
+            if (j >= npv - H) sl[ns++] = band_slot(npv, H, p, 3, npv + 1 - j, fl, i);
             for (int e = 0; e < ns; ++e) {
-              stg(a.edge_raw, sl[e], vals[fl]);
-              if (a.oc_phi) stg(a.edge_raw_c, sl[e], lows[fl]);
+              stg(a.band_raw, sl[e], vals[fl]);
+              if (a.oc_phi) stg(a.band_raw_c, sl[e], lows[fl]);
             }
           }
         }
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(kStageThreads) void stage_kernel_f32(MeshF32 m, Sta
 struct BoundaryArgs32 {
   float *phi, *u, *v;      // level to finish
   float *cphi, *cu, *cv;   // its compensation words (base state only, else NULL)
-  const float *edge_raw, *edge_raw_c;
+  const float *band_raw, *band_raw_c;  // raw band lines; residues for the base state
 };
 
 template <int DOF>
@@ -440,27 +441,19 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
   constexpr int H = DOF - 1;
   float *const phi = ba.phi, *const u = ba.u, *const v = ba.v;
   float *const cphi = ba.cphi, *const cu = ba.cu, *const cv = ba.cv;
-  const float *const edge_raw = ba.edge_raw, *const edge_raw_c = ba.edge_raw_c;
   const int npv = m.npv, nh = m.nh;
   const double dx = m.dx;
   const int nghost = 6 * 4 * H * npv, nedge = 12 * npv;
-  auto raw = [&](int q, int i, int j, double &ph, double &uu, double &vv) {
-    int side = -1, s = 0;
-    if (i == 1) { side = 0; s = j; }
-    else if (i == npv) { side = 1; s = j; }
-    else if (j == 1) { side = 2; s = i; }
-    else if (j == npv) { side = 3; s = i; }
-    if (side >= 0) {
-      const long long s0 = edge_slot(npv, q, side, 0, s), s1 = edge_slot(npv, q, side, 1, s),
-                      s2 = edge_slot(npv, q, side, 2, s);
-      ph = edge_raw[s0]; uu = edge_raw[s1]; vv = edge_raw[s2];
-      if (cphi) {
-        ph += (double)edge_raw_c[s0]; uu += (double)edge_raw_c[s1]; vv += (double)edge_raw_c[s2];
-      }
-    } else {
-      const long long g = pidx(nh, H, q, i, j);
-      ph = phi[g]; uu = u[g]; vv = v[g];
-      if (cphi) { ph += (double)cphi[g]; uu += (double)cu[g]; vv += (double)cv[g]; }
+  // raw value of band point (panel q, side, depth d, along index al): hi, plus
+  // lo for the base state
+  auto raw = [&](int q, int side, int d, int al, double &ph, double &uu, double &vv) {
+    const uint32_t s0 = band_slot(npv, H, q, side, d, 0, al), s1 = band_slot(npv, H, q, side, d, 1, al),
+                   s2 = band_slot(npv, H, q, side, d, 2, al);
+    ph = ldg(ba.band_raw, s0); uu = ldg(ba.band_raw, s1); vv = ldg(ba.band_raw, s2);
+    if (cphi) {
+      ph += (double)ldg(ba.band_raw_c, s0);
+      uu += (double)ldg(ba.band_raw_c, s1);
+      vv += (double)ldg(ba.band_raw_c, s2);
     }
   };
   auto store = [&](long long g, double ph, double uu, double vv) {
@@ -472,78 +465,76 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
       cv[g] = (float)(vv - (double)a2);
     }
   };
-  {
-    if (t < nghost) {  // one ghost point: (panel, side, depth k, s), s fastest
-      const int s = t % npv + 1, r = t / npv, k = r % H + 1, ps = r / H;
-      const int side = ps % 4, p = ps / 4 + 1;
-      const HaloSrc hs = kHalo[p - 1][side];
-      {
-        const int c = m.giref[(k - 1) * npv + (s - 1)];
-        const double cf = m.gcoef[(k - 1) * npv + (s - 1)];
-        const int nrm = hs.far ? npv - k : k + 1;
-        double qp[DOF], qz[DOF], qm[DOF];
-        for (int l = 0; l < DOF; ++l) {
-          const int tpos = (c - 1) * H + 1 + l;
-          const int al = hs.rev ? npv + 1 - tpos : tpos;
-          const int qi = hs.col ? nrm : al, qj = hs.col ? al : nrm;
-          double ph, uu, vv;
-          raw(hs.q, qi, qj, ph, uu, vv);
-          qp[l] = ph;
-          apply2(m.p2sp + 4 * pidx(nh, H, hs.q, qi, qj), uu, vv, qz[l], qm[l]);
-        }
-        int hi, hj;
-        if (side == 0) { hi = 1 - k; hj = s; }
-        else if (side == 1) { hi = npv + k; hj = s; }
-        else if (side == 2) { hi = s; hj = 1 - k; }
-        else { hi = s; hj = npv + k; }
-        const long long g = pidx(nh, H, p, hi, hj);
-        const double zh = ghost_interp<DOF>(qz, cf, dx), mh = ghost_interp<DOF>(qm, cf, dx);
-        double uh, vh;
-        apply2(m.sp2p + 4 * g, zh, mh, uh, vh);
-        store(g, ghost_interp<DOF>(qp, cf, dx), uh, vh);
-      }
-    } else if (t < nghost + nedge) {  // one point of one cube edge
-      const int s = (t - nghost) % npv + 1, e = (t - nghost) / npv;
-      if (s > 1 && s < npv) {
-        const EdgePair ep = kEdges[e];
-        const int sb = ep.rev ? npv + 1 - s : s;
-        int ia, ja, ib, jb;
-        edge_ij(npv, ep.sa, s, ia, ja);
-        edge_ij(npv, ep.sb, sb, ib, jb);
-        const long long ga = pidx(nh, H, ep.pa, ia, ja), gb = pidx(nh, H, ep.pb, ib, jb);
-        double pa, ua, va, pb, ub, vb;
-        raw(ep.pa, ia, ja, pa, ua, va);
-        raw(ep.pb, ib, jb, pb, ub, vb);
-        double za, ma, zb, mb;
-        apply2(m.p2sp + 4 * ga, ua, va, za, ma);
-        apply2(m.p2sp + 4 * gb, ub, vb, zb, mb);
-        const double ph = 0.5 * (pa + pb), zu = 0.5 * (za + zb), mu = 0.5 * (ma + mb);
-        double u1, v1, u2, v2;
-        apply2(m.sp2p + 4 * ga, zu, mu, u1, v1);
-        apply2(m.sp2p + 4 * gb, zu, mu, u2, v2);
-        store(ga, ph, u1, v1);
-        store(gb, ph, u2, v2);
-      }
-    } else {
-      const int vx = t - nghost - nedge;
-      double ph[3], z[3], mr[3];
-      long long gg[3];
-      for (int l = 0; l < 3; ++l) {
-        const VtxPt vp = kVerts[vx][l];
-        const int i = vp.ci ? npv : 1, j = vp.cj ? npv : 1;
-        double uu, vv;
-        raw(vp.p, i, j, ph[l], uu, vv);
-        gg[l] = pidx(nh, H, vp.p, i, j);
-        apply2(m.p2sp + 4 * gg[l], uu, vv, z[l], mr[l]);
-      }
-      const double pv = (ph[0] + ph[1] + ph[2]) / 3.0;
-      const double zv = (z[0] + z[1] + z[2]) / 3.0;
-      const double mv = (mr[0] + mr[1] + mr[2]) / 3.0;
-      for (int l = 0; l < 3; ++l) {
-        double uu, vv;
-        apply2(m.sp2p + 4 * gg[l], zv, mv, uu, vv);
-        store(gg[l], pv, uu, vv);
-      }
+  if (t < nghost) {  // one ghost point: (panel, side, depth k, s), s fastest
+    const int s = t % npv + 1, r = t / npv, k = r % H + 1, ps = r / H;
+    const int side = ps % 4, p = ps / 4 + 1;
+    const HaloSrc hs = kHalo[p - 1][side];
+    const int src = hs.col ? (hs.far ? 1 : 0) : (hs.far ? 3 : 2);
+    const int c = m.giref[(k - 1) * npv + (s - 1)];
+    const double cf = m.gcoef[(k - 1) * npv + (s - 1)];
+    double qp[DOF], qz[DOF], qm[DOF];
+    for (int l = 0; l < DOF; ++l) {
+      const int tpos = (c - 1) * H + 1 + l;
+      const int al = hs.rev ? npv + 1 - tpos : tpos;
+      double ph, uu, vv;
+      raw(hs.q, src, k + 1, al, ph, uu, vv);
+      qp[l] = ph;
+      apply2(m.p2sp_band + 4 * band_point(npv, H, hs.q, src, k + 1, al), uu, vv, qz[l], qm[l]);
+    }
+    int hi, hj;
+    if (side == 0) { hi = 1 - k; hj = s; }
+    else if (side == 1) { hi = npv + k; hj = s; }
+    else if (side == 2) { hi = s; hj = 1 - k; }
+    else { hi = s; hj = npv + k; }
+    const long long g = pidx(nh, H, p, hi, hj);
+    const double zh = ghost_interp<DOF>(qz, cf, dx), mh = ghost_interp<DOF>(qm, cf, dx);
+    double uh, vh;
+    apply2(m.sp2p_ghost + 4 * ghost_point(npv, H, p, side, k, s), zh, mh, uh, vh);
+    store(g, ghost_interp<DOF>(qp, cf, dx), uh, vh);
+  } else if (t < nghost + nedge) {  // one point of one cube edge
+    const int s = (t - nghost) % npv + 1, e = (t - nghost) / npv;
+    if (s > 1 && s < npv) {
+      const EdgePair ep = kEdges[e];
+      const int sb = ep.rev ? npv + 1 - s : s;
+      int ia, ja, ib, jb;
+      edge_ij(npv, ep.sa, s, ia, ja);
+      edge_ij(npv, ep.sb, sb, ib, jb);
+      const long long ga = pidx(nh, H, ep.pa, ia, ja), gb = pidx(nh, H, ep.pb, ib, jb);
+      double pa, ua, va, pb, ub, vb;
+      raw(ep.pa, ep.sa, 1, s, pa, ua, va);
+      raw(ep.pb, ep.sb, 1, sb, pb, ub, vb);
+      double za, ma, zb, mb;
+      apply2(m.p2sp_band + 4 * band_point(npv, H, ep.pa, ep.sa, 1, s), ua, va, za, ma);
+      apply2(m.p2sp_band + 4 * band_point(npv, H, ep.pb, ep.sb, 1, sb), ub, vb, zb, mb);
+      const double ph = 0.5 * (pa + pb), zu = 0.5 * (za + zb), mu = 0.5 * (ma + mb);
+      double u1, v1, u2, v2;
+      apply2(m.sp2p_edge + 4 * edge_point(npv, ep.pa, ep.sa, s), zu, mu, u1, v1);
+      apply2(m.sp2p_edge + 4 * edge_point(npv, ep.pb, ep.sb, sb), zu, mu, u2, v2);
+      store(ga, ph, u1, v1);
+      store(gb, ph, u2, v2);
+    }
+  } else {
+    const int vx = t - nghost - nedge;
+    double ph[3], z[3], mr[3];
+    long long gg[3];
+    uint32_t eg[3];
+    for (int l = 0; l < 3; ++l) {
+      const VtxPt vp = kVerts[vx][l];
+      const int i = vp.ci ? npv : 1, j = vp.cj ? npv : 1;
+      const int side = vp.ci ? 1 : 0;
+      double uu, vv;
+      raw(vp.p, side, 1, j, ph[l], uu, vv);
+      gg[l] = pidx(nh, H, vp.p, i, j);
+      eg[l] = edge_point(npv, vp.p, side, j);
+      apply2(m.p2sp_band + 4 * band_point(npv, H, vp.p, side, 1, j), uu, vv, z[l], mr[l]);
+    }
+    const double pv = (ph[0] + ph[1] + ph[2]) / 3.0;
+    const double zv = (z[0] + z[1] + z[2]) / 3.0;
+    const double mv = (mr[0] + mr[1] + mr[2]) / 3.0;
+    for (int l = 0; l < 3; ++l) {
+      double uu, vv;
+      apply2(m.sp2p_edge + 4 * eg[l], zv, mv, uu, vv);
+      store(gg[l], pv, uu, vv);
     }
   }
 }
@@ -572,19 +563,20 @@ static __global__ void pgref_kernel(MeshDev m, const double *phi_ref, float *pgr
   }
 }
 
-// FP64 edge-line copy of a finished level (to run the boundary pass on it).
-static __global__ void edge_copy_kernel(MeshDev m, const double *phi, const double *u, const double *v,
-                                 double *edge_raw) {
+// FP64 band copy of a finished level (to run the boundary pass on it).
+static __global__ void band_copy_kernel(MeshDev m, const double *phi, const double *u,
+                                        const double *v, double *band) {
   const int npv = m.npv, H = m.dof - 1;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < 6 * 4 * npv;
-       t += gridDim.x * blockDim.x) {
-    const int s = t % npv + 1, ps = t / npv, side = ps % 4, p = ps / 4 + 1;
+  const int n = 6 * 4 * (H + 1) * npv;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int al = t % npv + 1, r = t / npv, d = r % (H + 1) + 1, ps = r / (H + 1);
+    const int side = ps % 4, p = ps / 4 + 1;
     int i, j;
-    edge_ij(npv, side, s, i, j);
+    band_ij(npv, side, d, al, i, j);
     const long long g = pidx(m.nh, H, p, i, j);
-    edge_raw[edge_slot(npv, p, side, 0, s)] = phi[g];
-    edge_raw[edge_slot(npv, p, side, 1, s)] = u[g];
-    edge_raw[edge_slot(npv, p, side, 2, s)] = v[g];
+    band[band_slot(npv, H, p, side, d, 0, al)] = phi[g];
+    band[band_slot(npv, H, p, side, d, 1, al)] = u[g];
+    band[band_slot(npv, H, p, side, d, 2, al)] = v[g];
   }
 }
 

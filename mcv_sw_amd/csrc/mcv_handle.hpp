@@ -72,7 +72,7 @@ struct mcv_handle {
   size_t device_bytes = 0;     // their sizes summed
   double *lv[3][3] = {};       // level (Yn, S1, S2) x field (phi, u, v)
   double *acc[3] = {};
-  double *edge_raw = nullptr;
+  double *band_raw = nullptr;  // raw band lines written by the stage kernel
   double *cu0 = nullptr, *cv0 = nullptr;  // host contra winds until the first stage
   bool first_pending = true;
   double *partials = nullptr;
@@ -92,7 +92,7 @@ struct mcv_handle {
   float *lv32[3][3] = {};
   float *comp32[3] = {};
   float *acc32[3] = {};
-  float *edge32 = nullptr, *edge32c = nullptr;
+  float *band32 = nullptr, *band32c = nullptr;  // FP32 band lines and residues
   double *join[3] = {};  // FP64 reconstruction of Y_n for read-back and integrals
   mcv::MeshF32 mesh32{};
 };

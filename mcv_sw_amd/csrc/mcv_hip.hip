@@ -34,20 +34,21 @@ int fail(int code, const char *fmt, ...) {
 
 size_t footprint_bytes(const mcv_config *c) {
   const size_t H = c->dof - 1, npv = (size_t)c->n * H + 1, nh = npv + 2 * H;
-  const size_t np = 6 * nh * nh, nedge = 6 * 4 * 3 * npv, pad = 256;
+  const size_t np = 6 * nh * nh, nband = band_values(c->dof, (int)npv), pad = 256;
   const size_t ncell = 6 * (size_t)c->n * c->n;
   const size_t diag_blocks = std::min<size_t>((ncell + kDiagThreads - 1) / kDiagThreads, 1024);
   auto d = [&](size_t count) { return count * sizeof(double) + pad; };
   auto f = [&](size_t count) { return count * sizeof(float) + pad; };
   // static mesh data (panel-invariant metric terms counted on every panel)
-  size_t b = 5 * d(np) + d(np) + d(np) + 2 * d(4 * np) + d(H * npv) +
+  size_t b = 5 * d(np) + d(np) + d(np) + d(4 * np) + d(H * npv) +
              sizeof(int32_t) * H * npv + pad;
+  b += d(4 * 6 * 4 * (H + 1) * npv) + d(4 * 6 * 4 * H * npv) + d(4 * 6 * 4 * npv);  // band tables
   const bool fp32c = c->precision == MCV_FP32C;
   b += (fp32c ? 3 : 9) * d(np);                       // FP64 levels
   if (!fp32c) b += 3 * d(np);                          // RK4 accumulator
-  b += d(nedge) + 2 * d(np) + d(4 * diag_blocks) + d(2) + 6 * d(np);
+  b += d(nband) + 2 * d(np) + d(4 * diag_blocks) + d(2) + 6 * d(np);
   if (fp32c)
-    b += 3 * d(np) + 3 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) + 2 * f(nedge) + f(np);
+    b += 3 * d(np) + 3 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) + 2 * f(nband) + f(np);
   return b;
 }
 
@@ -57,13 +58,13 @@ StageArgs stage_args64(const mcv_handle *h, int in, int out, double c1) {
   a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
   a.acc_phiG = h->acc[0]; a.acc_u = h->acc[1]; a.acc_v = h->acc[2];
   a.o_phi = h->lv[out][0]; a.o_u = h->lv[out][1]; a.o_v = h->lv[out][2];
-  a.edge_raw = h->edge_raw;
+  a.band_raw = h->band_raw;
   a.c1 = c1;
   return a;
 }
 
 BoundaryArgs boundary_args64(const mcv_handle *h, int level) {
-  return BoundaryArgs{h->lv[level][0], h->lv[level][1], h->lv[level][2], h->edge_raw};
+  return BoundaryArgs{h->lv[level][0], h->lv[level][1], h->lv[level][2], h->band_raw};
 }
 
 StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1) {
@@ -76,8 +77,8 @@ StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1) {
   if (out == 0) {  // the step's last stage writes Y_n+1 with its compensation
     a.oc_phi = h->comp32[0]; a.oc_u = h->comp32[1]; a.oc_v = h->comp32[2];
   }
-  a.edge_raw = h->edge32;
-  a.edge_raw_c = h->edge32c;
+  a.band_raw = h->band32;
+  a.band_raw_c = h->band32c;
   a.c1 = c1;
   return a;
 }
@@ -87,8 +88,8 @@ BoundaryArgs32 boundary_args32(const mcv_handle *h, int level) {
   BoundaryArgs32 b{};
   b.phi = h->lv32[level][0]; b.u = h->lv32[level][1]; b.v = h->lv32[level][2];
   if (base) { b.cphi = h->comp32[0]; b.cu = h->comp32[1]; b.cv = h->comp32[2]; }
-  b.edge_raw = h->edge32;
-  b.edge_raw_c = h->edge32c;
+  b.band_raw = h->band32;
+  b.band_raw_c = h->band32c;
   return b;
 }
 
@@ -366,16 +367,16 @@ int dev_alloc_t(mcv_handle *h, T **out, size_t count) {
 // stage sees the handed-over halo and edges as the FP64 path does.
 int setup32(mcv_handle *h) {
   const long long np = h->np;
-  const size_t nedge = (size_t)6 * 4 * 3 * h->npv;
-  const int threads = 6 * 4 * h->npv;
+  const size_t nband = band_values(h->cfg.dof, h->npv);
+  const int threads = 6 * 4 * h->cfg.dof * h->npv;
   int rc;
   for (int f = 0; f < 3; ++f) {
     if ((rc = dev_alloc(h, &h->join[f], np))) return rc;
     HIP_TRY(hipMemcpyAsync(h->join[f], h->lv[0][f], np * sizeof(double),
                            hipMemcpyDeviceToDevice, h->stream));
   }
-  hipLaunchKernelGGL(edge_copy_kernel, dim3((threads + 255) / 256), dim3(256), 0, h->stream,
-                     h->mesh, h->lv[0][0], h->lv[0][1], h->lv[0][2], h->edge_raw);
+  hipLaunchKernelGGL(band_copy_kernel, dim3((threads + 255) / 256), dim3(256), 0, h->stream,
+                     h->mesh, h->lv[0][0], h->lv[0][1], h->lv[0][2], h->band_raw);
   HIP_TRY(hipGetLastError());
   if ((rc = boundary(h, 0))) return rc;
   h->first_pending = false;  // the FP32C stages form the contravariant winds themselves
@@ -400,7 +401,7 @@ int setup32(mcv_handle *h) {
     if ((rc = dev_alloc_t(h, &h->comp32[f], np))) return rc;
     if ((rc = dev_alloc_t(h, &h->acc32[f], np))) return rc;
   }
-  if ((rc = dev_alloc_t(h, &h->edge32, nedge)) || (rc = dev_alloc_t(h, &h->edge32c, nedge)))
+  if ((rc = dev_alloc_t(h, &h->band32, nband)) || (rc = dev_alloc_t(h, &h->band32c, nband)))
     return rc;
   float *pgref;
   if ((rc = dev_alloc_t(h, &pgref, np))) return rc;
@@ -536,8 +537,40 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   m.phi_s = d;
   if ((rc = upload(h, &d, p2sp.data(), 4 * np))) return rc;
   m.p2sp = d;
-  if ((rc = upload(h, &d, sp2p.data(), 4 * np))) return rc;
-  m.sp2p = d;
+  {
+    // line-major matrix tables of the boundary pass (band, ghost and edge points)
+    const int npv = h->npv;
+    std::vector<double> pb(4 * (size_t)6 * 4 * (H + 1) * npv), sg(4 * (size_t)6 * 4 * H * npv),
+        se(4 * (size_t)6 * 4 * npv);
+    for (int p = 1; p <= 6; ++p)
+      for (int side = 0; side < 4; ++side)
+        for (int al = 1; al <= npv; ++al) {
+          for (int dd = 1; dd <= H + 1; ++dd) {
+            int i, j;
+            band_ij(npv, side, dd, al, i, j);
+            std::memcpy(&pb[4 * (size_t)band_point(npv, H, p, side, dd, al)],
+                        &p2sp[4 * pidx(h->nh, H, p, i, j)], 4 * sizeof(double));
+            if (dd == 1)
+              std::memcpy(&se[4 * (size_t)edge_point(npv, p, side, al)],
+                          &sp2p[4 * pidx(h->nh, H, p, i, j)], 4 * sizeof(double));
+          }
+          for (int k = 1; k <= H; ++k) {
+            int hi, hj;
+            if (side == 0) { hi = 1 - k; hj = al; }
+            else if (side == 1) { hi = npv + k; hj = al; }
+            else if (side == 2) { hi = al; hj = 1 - k; }
+            else { hi = al; hj = npv + k; }
+            std::memcpy(&sg[4 * (size_t)ghost_point(npv, H, p, side, k, al)],
+                        &sp2p[4 * pidx(h->nh, H, p, hi, hj)], 4 * sizeof(double));
+          }
+        }
+    if ((rc = upload(h, &d, pb.data(), pb.size()))) return rc;
+    m.p2sp_band = d;
+    if ((rc = upload(h, &d, sg.data(), sg.size()))) return rc;
+    m.sp2p_ghost = d;
+    if ((rc = upload(h, &d, se.data(), se.size()))) return rc;
+    m.sp2p_edge = d;
+  }
   if ((rc = upload(h, &d, mesh->ghost_coef, (size_t)H * h->npv))) return rc;
   m.gcoef = d;
   {
@@ -558,7 +591,7 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   if (!h->fp32c)
     for (int f = 0; f < 3; ++f)
       if ((rc = dev_alloc(h, &h->acc[f], np))) return rc;
-  if ((rc = dev_alloc(h, &h->edge_raw, (size_t)6 * 4 * 3 * h->npv))) return rc;
+  if ((rc = dev_alloc(h, &h->band_raw, band_values(cfg->dof, h->npv)))) return rc;
   HIP_TRY(hipMemcpy(h->lv[0][0], st->phi, np * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->lv[0][1], st->u, np * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->lv[0][2], st->v, np * sizeof(double), hipMemcpyHostToDevice));
@@ -730,7 +763,7 @@ int mcv_hip_autotune(mcv_handle *h, int32_t reps, int32_t *tile, double *ms_per_
   a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
   a.acc_phiG = h->acc[0]; a.acc_u = h->acc[1]; a.acc_v = h->acc[2];
   a.o_phi = h->lv[1][0]; a.o_u = h->lv[1][1]; a.o_v = h->lv[1][2];
-  a.edge_raw = h->edge_raw;
+  a.band_raw = h->band_raw;
   if (h->first_pending) { a.cu0 = h->cu0; a.cv0 = h->cv0; }
   a.c1 = 0.5 * h->cfg.dt;
   const int kind = h->cfg.scheme == MCV_RK4 ? kRK4S2 : kRK3S2;

@@ -19,33 +19,11 @@
 
 namespace mcv {
 
-// Raw (pre-averaging) phi, u, v of interior point (q, i, j) after a stage:
-// edge-line points come from the raw copy, the rest from the level itself.
-__device__ __forceinline__ void raw_point(const MeshDev &m, const BoundaryArgs &a, int q, int i,
-                                          int j, double &phi, double &u, double &v) {
-  const int npv = m.npv;
-  int side = -1, s = 0;
-  if (i == 1) { side = 0; s = j; }
-  else if (i == npv) { side = 1; s = j; }
-  else if (j == 1) { side = 2; s = i; }
-  else if (j == npv) { side = 3; s = i; }
-  if (side >= 0) {
-    phi = a.edge_raw[edge_slot(npv, q, side, 0, s)];
-    u = a.edge_raw[edge_slot(npv, q, side, 1, s)];
-    v = a.edge_raw[edge_slot(npv, q, side, 2, s)];
-  } else {
-    const long long g = pidx(m.nh, m.dof - 1, q, i, j);
-    phi = a.phi[g];
-    u = a.u[g];
-    v = a.v[g];
-  }
-}
-
-// correct_bdy_ghost (time_scheme_mod.F90:125-142) on one level: one thread per
-// (panel, side, s), plus 8 threads for the cube vertices.
 // Work item t of correct_bdy_ghost on one level, t in [0, boundary_items()):
-// one ghost point, one edge point or one cube vertex per item, so that each
-// item is a short chain of loads and every item is independent.
+// one ghost point, one edge point or one cube vertex per item.  Every value is
+// read from the band (the raw lines the stage kernel wrote) and the
+// line-major matrix tables, contiguously along the lines; the items are
+// independent, each a short chain of loads.
 template <int DOF>
 __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryArgs &a,
                                               const int t) {
@@ -58,60 +36,61 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
     const int s = t % npv + 1, r = t / npv, k = r % H + 1, ps = r / H;
     const int side = ps % 4, p = ps / 4 + 1;
     const HaloSrc hs = kHalo[p - 1][side];
-    // Ghost point at depth k (ghost_mod.f90:154-227), sphere winds through
-    // P2SP at the neighbour's points and back through SP2P at the ghost point
-    // (spatial_operators_mod.F90:408-492).
-    {
-      const int c = m.giref[(k - 1) * npv + (s - 1)];
-      const double cf = m.gcoef[(k - 1) * npv + (s - 1)];
-      const int nrm = hs.far ? npv - k : k + 1;
-      double qp[DOF], qz[DOF], qm[DOF];
+    // Ghost point at depth k (ghost_mod.f90:154-227): the neighbour's line at
+    // normal index k+1 (or nPV-k) is band depth k+1 of the side it is parallel
+    // to; sphere winds through P2SP there and back through SP2P at the ghost
+    // point (spatial_operators_mod.F90:408-492).
+    const int src = hs.col ? (hs.far ? 1 : 0) : (hs.far ? 3 : 2);
+    const int c = m.giref[(k - 1) * npv + (s - 1)];
+    const double cf = m.gcoef[(k - 1) * npv + (s - 1)];
+    double qp[DOF], qz[DOF], qm[DOF];
 #pragma unroll
-      for (int l = 0; l < DOF; ++l) {
-        const int tpos = (c - 1) * H + 1 + l;
-        const int al = hs.rev ? npv + 1 - tpos : tpos;
-        const int qi = hs.col ? nrm : al, qj = hs.col ? al : nrm;
-        double phi, u, v;
-        raw_point(m, a, hs.q, qi, qj, phi, u, v);
-        qp[l] = phi;
-        apply2(m.p2sp + 4 * pidx(nh, H, hs.q, qi, qj), u, v, qz[l], qm[l]);
-      }
-      int hi, hj;
-      if (side == 0) { hi = 1 - k; hj = s; }
-      else if (side == 1) { hi = npv + k; hj = s; }
-      else if (side == 2) { hi = s; hj = 1 - k; }
-      else { hi = s; hj = npv + k; }
-      const long long g = pidx(nh, H, p, hi, hj);
-      const double zh = ghost_interp<DOF>(qz, cf, dx), mh = ghost_interp<DOF>(qm, cf, dx);
-      a.phi[g] = ghost_interp<DOF>(qp, cf, dx);
-      double uh, vh;
-      apply2(m.sp2p + 4 * g, zh, mh, uh, vh);
-      a.u[g] = uh;
-      a.v[g] = vh;
+    for (int l = 0; l < DOF; ++l) {
+      const int tpos = (c - 1) * H + 1 + l;
+      const int al = hs.rev ? npv + 1 - tpos : tpos;
+      const double ph = ldg(a.band_raw, band_slot(npv, H, hs.q, src, k + 1, 0, al));
+      const double u = ldg(a.band_raw, band_slot(npv, H, hs.q, src, k + 1, 1, al));
+      const double v = ldg(a.band_raw, band_slot(npv, H, hs.q, src, k + 1, 2, al));
+      qp[l] = ph;
+      apply2(m.p2sp_band + 4 * band_point(npv, H, hs.q, src, k + 1, al), u, v, qz[l], qm[l]);
     }
+    int hi, hj;
+    if (side == 0) { hi = 1 - k; hj = s; }
+    else if (side == 1) { hi = npv + k; hj = s; }
+    else if (side == 2) { hi = s; hj = 1 - k; }
+    else { hi = s; hj = npv + k; }
+    const long long g = pidx(nh, H, p, hi, hj);
+    const double zh = ghost_interp<DOF>(qz, cf, dx), mh = ghost_interp<DOF>(qm, cf, dx);
+    a.phi[g] = ghost_interp<DOF>(qp, cf, dx);
+    double uh, vh;
+    apply2(m.sp2p_ghost + 4 * ghost_point(npv, H, p, side, k, s), zh, mh, uh, vh);
+    a.u[g] = uh;
+    a.v[g] = vh;
   } else if (t < nghost + nedge) {
     // Edge averaging (spatial_operators_mod.F90:298-345), vertices excluded.
     const int s = (t - nghost) % npv + 1, e = (t - nghost) / npv;
     if (s > 1 && s < npv) {
       const EdgePair ep = kEdges[e];
       const int sb = ep.rev ? npv + 1 - s : s;
-      const double pa = a.edge_raw[edge_slot(npv, ep.pa, ep.sa, 0, s)];
-      const double ua = a.edge_raw[edge_slot(npv, ep.pa, ep.sa, 1, s)];
-      const double va = a.edge_raw[edge_slot(npv, ep.pa, ep.sa, 2, s)];
-      const double pb = a.edge_raw[edge_slot(npv, ep.pb, ep.sb, 0, sb)];
-      const double ub = a.edge_raw[edge_slot(npv, ep.pb, ep.sb, 1, sb)];
-      const double vb = a.edge_raw[edge_slot(npv, ep.pb, ep.sb, 2, sb)];
+      const double pa = ldg(a.band_raw, band_slot(npv, H, ep.pa, ep.sa, 1, 0, s));
+      const double ua = ldg(a.band_raw, band_slot(npv, H, ep.pa, ep.sa, 1, 1, s));
+      const double va = ldg(a.band_raw, band_slot(npv, H, ep.pa, ep.sa, 1, 2, s));
+      const double pb = ldg(a.band_raw, band_slot(npv, H, ep.pb, ep.sb, 1, 0, sb));
+      const double ub = ldg(a.band_raw, band_slot(npv, H, ep.pb, ep.sb, 1, 1, sb));
+      const double vb = ldg(a.band_raw, band_slot(npv, H, ep.pb, ep.sb, 1, 2, sb));
+      const uint32_t ba = band_point(npv, H, ep.pa, ep.sa, 1, s);
+      const uint32_t bb = band_point(npv, H, ep.pb, ep.sb, 1, sb);
       int ia, ja, ib, jb;
       edge_ij(npv, ep.sa, s, ia, ja);
       edge_ij(npv, ep.sb, sb, ib, jb);
       const long long ga = pidx(nh, H, ep.pa, ia, ja), gb = pidx(nh, H, ep.pb, ib, jb);
       double za, ma, zb, mb;
-      apply2(m.p2sp + 4 * ga, ua, va, za, ma);
-      apply2(m.p2sp + 4 * gb, ub, vb, zb, mb);
+      apply2(m.p2sp_band + 4 * ba, ua, va, za, ma);
+      apply2(m.p2sp_band + 4 * bb, ub, vb, zb, mb);
       const double ph = 0.5 * (pa + pb), zu = 0.5 * (za + zb), mu = 0.5 * (ma + mb);
       double u1, v1, u2, v2;
-      apply2(m.sp2p + 4 * ga, zu, mu, u1, v1);
-      apply2(m.sp2p + 4 * gb, zu, mu, u2, v2);
+      apply2(m.sp2p_edge + 4 * edge_point(npv, ep.pa, ep.sa, s), zu, mu, u1, v1);
+      apply2(m.sp2p_edge + 4 * edge_point(npv, ep.pb, ep.sb, sb), zu, mu, u2, v2);
       a.phi[ga] = ph; a.u[ga] = u1; a.v[ga] = v1;
       a.phi[gb] = ph; a.u[gb] = u2; a.v[gb] = v2;
     }
@@ -120,16 +99,18 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
     const int vx = t - nghost - nedge;
     double ph[3], z[3], mr[3];
     long long gg[3];
+    uint32_t eg[3];
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       const VtxPt vp = kVerts[vx][l];
       const int i = vp.ci ? npv : 1, j = vp.cj ? npv : 1;
-      const int side = vp.ci ? 1 : 0;  // any edge through the corner holds its raw copy
-      ph[l] = a.edge_raw[edge_slot(npv, vp.p, side, 0, j)];
-      const double u = a.edge_raw[edge_slot(npv, vp.p, side, 1, j)];
-      const double v = a.edge_raw[edge_slot(npv, vp.p, side, 2, j)];
+      const int side = vp.ci ? 1 : 0;  // either edge through the corner holds its raw value
+      ph[l] = ldg(a.band_raw, band_slot(npv, H, vp.p, side, 1, 0, j));
+      const double u = ldg(a.band_raw, band_slot(npv, H, vp.p, side, 1, 1, j));
+      const double v = ldg(a.band_raw, band_slot(npv, H, vp.p, side, 1, 2, j));
       gg[l] = pidx(nh, H, vp.p, i, j);
-      apply2(m.p2sp + 4 * gg[l], u, v, z[l], mr[l]);
+      eg[l] = edge_point(npv, vp.p, side, j);
+      apply2(m.p2sp_band + 4 * band_point(npv, H, vp.p, side, 1, j), u, v, z[l], mr[l]);
     }
     const double pv = (ph[0] + ph[1] + ph[2]) / 3.0;
     const double zv = (z[0] + z[1] + z[2]) / 3.0;
@@ -137,7 +118,7 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       double u, v;
-      apply2(m.sp2p + 4 * gg[l], zv, mv, u, v);
+      apply2(m.sp2p_edge + 4 * eg[l], zv, mv, u, v);
       a.phi[gg[l]] = pv;
       a.u[gg[l]] = u;
       a.v[gg[l]] = v;

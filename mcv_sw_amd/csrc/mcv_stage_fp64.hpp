@@ -6,8 +6,8 @@
 // the Riemann derivative at every cell end of the tile once (the reference
 // computes each interface twice, from both cells).  Phase 2 forms the
 // interior-point derivatives, the vorticity/Coriolis tendencies and the RK
-// update, and writes the raw updated interior plus a raw copy of the panel's
-// edge lines for the boundary pass.  Every loop runs over compile-time
+// update, and writes the raw updated interior plus a raw copy of the H+1
+// lines nearest each panel side (the band) for the boundary pass.  Every loop runs over compile-time
 // extents, so no index needs a run-time integer division.
 #pragma once
 
@@ -332,15 +332,15 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
         stg(a.o_phi, g, phi);
         stg(a.o_u, g, u);
         stg(a.o_v, g, v);
-        // raw copy of edge-line points for the boundary pass
-        if (i == 1 || i == npv || j == 1 || j == npv) {
+        // raw copy of the lines nearest each side for the boundary pass
+        if (i <= H + 1 || i >= npv - H || j <= H + 1 || j >= npv - H) {
           const double vals[3] = {phi, u, v};
 #pragma unroll
           for (int fl = 0; fl < 3; ++fl) {
-            if (i == 1) stg(a.edge_raw, edge_slot32(npv, p, 0, fl, j), vals[fl]);
-            if (i == npv) stg(a.edge_raw, edge_slot32(npv, p, 1, fl, j), vals[fl]);
-            if (j == 1) stg(a.edge_raw, edge_slot32(npv, p, 2, fl, i), vals[fl]);
-            if (j == npv) stg(a.edge_raw, edge_slot32(npv, p, 3, fl, i), vals[fl]);
+            if (i <= H + 1) stg(a.band_raw, band_slot(npv, H, p, 0, i, fl, j), vals[fl]);
+            if (i >= npv - H) stg(a.band_raw, band_slot(npv, H, p, 1, npv + 1 - i, fl, j), vals[fl]);
+            if (j <= H + 1) stg(a.band_raw, band_slot(npv, H, p, 2, j, fl, i), vals[fl]);
+            if (j >= npv - H) stg(a.band_raw, band_slot(npv, H, p, 3, npv + 1 - j, fl, i), vals[fl]);
           }
         }
       }
