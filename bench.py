@@ -145,7 +145,29 @@ def cpu_baseline(mesh_fn, steps=2):
                       f"restatement (gcc -O2, one core), {sec:.2f} s"}
 
 
-def run_single(capi, m, st, precision, steps, warmup):
+def time_history(capi, hostlib, s, m):
+    """The output block of MCV_SW.F90:62-70 on the device at full size:
+    cell averages alone, the record fields (device + read-back), and one
+    whole history record written to a file (history_init first)."""
+    r = {}
+    t = time.perf_counter()
+    s.cell_averages()
+    r["cell_averages_ms"] = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    s.history_fields()
+    r["history_fields_ms"] = (time.perf_counter() - t) * 1e3
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "mcv_output.nc")
+        capi.history_init(path, DOF, N, DT, hostlib.cells(DOF, N), m["lonP"], m["latP"],
+                          m["phi_s"])
+        t = time.perf_counter()
+        s.history_write(path, 2)
+        r["history_record_write_ms"] = (time.perf_counter() - t) * 1e3
+        r["history_record_bytes"] = os.path.getsize(path)
+    return r
+
+
+def run_single(capi, m, st, precision, steps, warmup, history=None):
     """Times `steps` steps after warm-up and autotuning; returns the timing
     record and the final state after warmup + 2 + steps + nt steps."""
     cells = 6 * N * N
@@ -163,10 +185,13 @@ def run_single(capi, m, st, precision, steps, warmup):
         stage_ms, bnd_ms = s.step_timed(nt)
         out = s.sync(fields=("phi", "u", "v"))
         out["mass0"] = mass0
+        hist = history(s) if history else None
         rec = {"value": cells * steps / wall, "ms_per_step": wall * 1e3 / steps,
                "stage_kernel_ms_per_step": stage_ms / nt,
                "boundary_kernel_ms_per_step": bnd_ms / nt, "tile_variant": tile,
                "tile_variant_ms_per_stage": tile_ms, "steps_compared": warmup + 2 + steps + nt}
+        if hist:
+            rec["history"] = hist
     return rec, out
 
 
@@ -338,7 +363,8 @@ def main():
     st = hostlib.jet(m, u0=U0, h0=H0, lmax=LMAX, seed=SEED, rms_h=RMS)
     log(f"mesh + initial state: {time.time() - t0:.1f} s")
     cells = 6 * N * N
-    rec64, out64 = run_single(capi, m, st, capi.FP64, args.steps, args.warmup)
+    rec64, out64 = run_single(capi, m, st, capi.FP64, args.steps, args.warmup,
+                              history=lambda s: time_history(capi, hostlib, s, m))
     log(f"FP64: {rec64['ms_per_step']:.3f} ms/step")
     nt_launches = 4 * max(10, args.steps // 5)
     bpcs = words_per_point(SCHEME) * 8 * (DOF - 1) ** 2  # bytes per cell-step
@@ -376,6 +402,7 @@ def main():
                      "tile_variant_ms_per_stage": rec64["tile_variant_ms_per_stage"],
                      "boundary_kernel_ms_per_step": rec64["boundary_kernel_ms_per_step"]},
         "mass_drift_fp64": float((out64["mass"] - out64["mass0"]) / out64["mass0"]),
+        "history": rec64.pop("history"),
     }
     if not args.no_fp32c:
         rec32, out32 = run_single(capi, m, st, capi.FP32C, args.steps, args.warmup)

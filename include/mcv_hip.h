@@ -208,6 +208,62 @@ int mcv_batch_run_timed(mcv_batch *b, int32_t nstages, double *stage_ms,
 /* Releases the batch and every solve in it.  NULL is accepted. */
 int mcv_batch_free(mcv_batch *b);
 
+/* ---------------------------------------------------------------------------
+ * Diagnostics and history output (output_mod.F90, diag_mod.F90:69-93).
+ * The history file is the reference's mcv_output.nc: netCDF classic format
+ * with 64-bit offsets (NF90_64BIT_OFFSET), the same dimensions, variables,
+ * attributes and record layout, written without libnetcdf.
+ * ------------------------------------------------------------------------- */
+
+/* Global attributes of history_init (output_mod.F90:86-106). */
+typedef struct mcv_history_info {
+  int32_t dof;      /* MCV_ORDER */
+  int32_t n;        /* cells per panel edge */
+  int32_t xhalo;    /* halo cells (must be 1: the library's arrays carry H halo points) */
+  int32_t yhalo;
+  int32_t case_num;
+  double dx, dy;    /* radians (written in degrees, x R2D, as the reference does) */
+  double dt;        /* s */
+} mcv_history_info;
+
+/* One history record (the arguments of history_write_stat,
+ * output_mod.F90:200-212).  Point fields (nh, nh, 6); cell fields
+ * (n+2, n+2, 6), the halo ring of cells zero (calc_VIA). */
+typedef struct mcv_history_fields {
+  double *u, *v, *contraU, *contraV, *phi, *phi_t, *zonal, *meridional;
+  double *uC, *vC, *phiC;
+} mcv_history_fields;
+
+/* Replaces history_init (output_mod.F90:13-166, called at MCV_SW.F90:49):
+ * creates (clobbers) `path` and writes the static variables.  Coordinates
+ * in radians, (n+2, n+2, 6) for cells and (nh, nh, 6) for points; areaCell
+ * (n+2, n+2, 6); phi_s (nh, nh, 6).  Host only (no device needed). */
+int mcv_history_init(const char *path, const mcv_history_info *info, const double *lonC,
+                     const double *latC, const double *lonP, const double *latP,
+                     const double *areaCell, const double *phi_s);
+
+/* Replaces history_write_stat (output_mod.F90:168-223) for host arrays:
+ * writes record `time_slot` (1-based) of an existing history file; the
+ * array sizes are the file's dimensions.  Used for the first record, which
+ * the reference writes from the initial state (MCV_SW.F90:50). */
+int mcv_history_write(const char *path, int32_t time_slot, const mcv_history_fields *rec);
+
+/* calc_VIA (diag_mod.F90:69-93) of the current u, v, phi on the device, FP64,
+ * in the reference's summation order: (n+2, n+2, 6) each, halo ring zero.
+ * Blocks. */
+int mcv_hip_cell_averages(mcv_handle *h, double *uC, double *vC, double *phiC);
+
+/* The record fields the output block of MCV_SW.F90:62-70 produces after a
+ * time step (addFillValue, convert_wind_P2SP, calc_VIA x 3, phi + phi_s),
+ * computed on the device; any pointer in *out may be NULL.  Halo corners:
+ * u, v and phi hold FillValue, zonal/meridional and contraU/V are formed
+ * from those FillValues.  MCV_ESTATE before the first step.  Blocks. */
+int mcv_hip_history_fields(mcv_handle *h, const mcv_history_fields *out);
+
+/* The output block of MCV_SW.F90:62-70 as one call: mcv_hip_history_fields
+ * into host staging, then mcv_history_write of record `time_slot`. */
+int mcv_hip_history_write(mcv_handle *h, const char *path, int32_t time_slot);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *mcv_hip_last_error(void);
 

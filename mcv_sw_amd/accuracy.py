@@ -3,7 +3,9 @@
 Runs one initial state twice on the GPU, FP64 (the reference's arithmetic bit
 for bit, so the FP64 trajectory is the oracle's) and FP32C, side by side, and
 reports at the requested simulated days the relative L2 and Linf distance of
-h, u and v over the owned points, and both runs' mass drift.
+h, u and v over the owned points, the area-weighted L1/L2/LInf of the cell
+averages (doc/orderCheck.m:60-82, device calc_VIA) of h, u and v, and both
+runs' mass drift.
 
     python -m mcv_sw_amd.accuracy --dof 4 --n 384 --dt 10 --days 1,5,10
 
@@ -24,7 +26,7 @@ import time
 
 import numpy as np
 
-from . import capi, hostlib
+from . import capi, diagnostics, hostlib
 
 
 def owned(a, dof, n):
@@ -57,25 +59,29 @@ def main(argv=None):
     ap.add_argument("--rms", type=float, default=60.0)
     ap.add_argument("--chunk-hours", type=float, default=6.0)
     ap.add_argument("--yardstick", action="store_true")
+    ap.add_argument("--lib", default=None, help="C-ABI library (default: the in-tree HIP build)")
     args = ap.parse_args(argv)
+    lib = capi.load(args.lib)
 
     def log(msg):
         print(f"[accuracy] {msg}", file=sys.stderr, flush=True)
 
     t0 = time.time()
     m = hostlib.mesh(args.dof, args.n)
+    area = hostlib.cells(args.dof, args.n)["areaCell"]
     st = hostlib.jet(m, u0=args.u0, h0=args.h0, lmax=args.lmax, seed=args.seed, rms_h=args.rms)
     log(f"setup {time.time() - t0:.1f} s")
     days = [float(d) for d in args.days.split(",")]
     targets = [int(round(d * 86400.0 / args.dt)) for d in days]
     chunk = max(1, int(round(args.chunk_hours * 3600.0 / args.dt)))
-    solvers = [capi.Solver(args.dof, args.n, args.dt, m, st, scheme=args.scheme),
+    solvers = [capi.Solver(args.dof, args.n, args.dt, m, st, scheme=args.scheme, lib=lib),
                capi.Solver(args.dof, args.n, args.dt, m, st, scheme=args.scheme,
-                           precision=capi.FP32C)]
+                           precision=capi.FP32C, lib=lib)]
     if args.yardstick:
         r32 = {k: st[k].astype(np.float32).astype(np.float64) for k in ("phi", "zonal", "merid")}
         sy = hostlib.handoff(m, r32["phi"], r32["zonal"], r32["merid"])
-        solvers.append(capi.Solver(args.dof, args.n, args.dt, m, sy, scheme=args.scheme))
+        solvers.append(capi.Solver(args.dof, args.n, args.dt, m, sy, scheme=args.scheme,
+                                   lib=lib))
         del sy
     del st
     try:
@@ -97,6 +103,11 @@ def main(argv=None):
             rec = {"day": day, "steps": done, "dof": args.dof, "n": args.n, "dt_s": args.dt,
                    "scheme": args.scheme}
             rec.update(compare(out[0], out[1], args.dof, args.n))
+            cav = [s.cell_averages() for s in solvers[:2]]  # (uC, vC, phiC) each
+            rec["ordercheck_fp32c"] = {
+                name: dict(zip(("L1", "L2", "LInf"),
+                               diagnostics.norms(cav[1][k], cav[0][k], area)))
+                for name, k in (("h", 2), ("u", 0), ("v", 1))}
             rec["mass_drift_fp32c"] = (out[1]["mass"] - mass0[1]) / mass0[1]
             rec["mass_drift_fp64"] = (out[0]["mass"] - mass0[0]) / mass0[0]
             if args.yardstick:

@@ -23,7 +23,7 @@ HIP_FLAGS = [
 ]
 
 
-HIP_SOURCES = ["mcv_hip.hip", "mcv_batch.hip", "mcv_probe.hip"]
+HIP_SOURCES = ["mcv_hip.hip", "mcv_batch.hip", "mcv_probe.hip", "mcv_netcdf.cpp"]
 
 
 def build_hip(verbose=False):

@@ -50,6 +50,20 @@ class State(ctypes.Structure):
 
 STATE_FIELDS = ("phi", "phiG", "u", "v", "contraU", "contraV")
 
+# one history record (history_write_stat, output_mod.F90:200-212)
+HISTORY_POINT_FIELDS = ("u", "v", "contraU", "contraV", "phi", "phi_t", "zonal", "meridional")
+HISTORY_CELL_FIELDS = ("uC", "vC", "phiC")
+
+
+class HistoryInfo(ctypes.Structure):
+    _fields_ = [("dof", ctypes.c_int32), ("n", ctypes.c_int32), ("xhalo", ctypes.c_int32),
+                ("yhalo", ctypes.c_int32), ("case_num", ctypes.c_int32), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("dt", ctypes.c_double)]
+
+
+class HistoryFields(ctypes.Structure):
+    _fields_ = [(k, _D) for k in HISTORY_POINT_FIELDS + HISTORY_CELL_FIELDS]
+
 
 class MCVError(RuntimeError):
     def __init__(self, code, msg):
@@ -108,12 +122,20 @@ def load(path=None):
     L.mcv_batch_set_graphs.argtypes = [B, ctypes.c_int32]
     L.mcv_batch_run_timed.argtypes = [B, ctypes.c_int32, _D, _D]
     L.mcv_batch_free.argtypes = [B]
+    L.mcv_history_init.argtypes = [ctypes.c_char_p, ctypes.POINTER(HistoryInfo)] + [_D] * 6
+    L.mcv_history_write.argtypes = [ctypes.c_char_p, ctypes.c_int32,
+                                    ctypes.POINTER(HistoryFields)]
+    L.mcv_hip_cell_averages.argtypes = [H, _D, _D, _D]
+    L.mcv_hip_history_fields.argtypes = [H, ctypes.POINTER(HistoryFields)]
+    L.mcv_hip_history_write.argtypes = [H, ctypes.c_char_p, ctypes.c_int32]
     for f in ("mcv_hip_init", "mcv_hip_step", "mcv_hip_sync", "mcv_hip_sphere_winds",
               "mcv_hip_tendency", "mcv_hip_steps_done", "mcv_hip_step_timed", "mcv_hip_free",
               "mcv_hip_footprint", "mcv_hip_device_bytes", "mcv_batch_create", "mcv_batch_add",
               "mcv_batch_remove", "mcv_batch_run", "mcv_batch_poll", "mcv_batch_sync",
               "mcv_batch_handle", "mcv_batch_autotune", "mcv_batch_set_tile",
-              "mcv_batch_set_graphs", "mcv_batch_run_timed", "mcv_batch_free"):
+              "mcv_batch_set_graphs", "mcv_batch_run_timed", "mcv_batch_free",
+              "mcv_history_init", "mcv_history_write", "mcv_hip_cell_averages",
+              "mcv_hip_history_fields", "mcv_hip_history_write"):
         getattr(L, f).restype = ctypes.c_int
     _libs[path] = L
     return L
@@ -171,6 +193,44 @@ def _read_state(call, nh, fields, integrals):
     return rc, out
 
 
+def _history_struct(rec, nh, nc):
+    """HistoryFields over the arrays of `rec` (a dict; missing fields NULL);
+    returns the struct and the arrays it points into."""
+    keep = {}
+    for k in HISTORY_POINT_FIELDS + HISTORY_CELL_FIELDS:
+        if k in rec and rec[k] is not None:
+            a = np.ascontiguousarray(rec[k], dtype=np.float64)
+            want = (6, nh, nh) if k in HISTORY_POINT_FIELDS else (6, nc, nc)
+            if a.shape != want:
+                raise ValueError(f"{k}: shape {a.shape}, expected {want}")
+            keep[k] = a
+    f = HistoryFields(*[_ptr(keep[k]) if k in keep else None
+                        for k in HISTORY_POINT_FIELDS + HISTORY_CELL_FIELDS])
+    return f, keep
+
+
+def history_init(path, dof, n, dt, cells, lonP, latP, phi_s, case_num=0, dx=None, lib=None):
+    """history_init (output_mod.F90:13-166): create the history file with the
+    static variables.  `cells` is hostlib.cells(dof, n); lonP, latP in radians."""
+    L = load(lib)
+    dxr = (np.pi / 2) / n if dx is None else dx
+    info = HistoryInfo(dof, n, 1, 1, case_num, dxr, dxr, float(dt))
+    arrs = [np.ascontiguousarray(a, dtype=np.float64)
+            for a in (cells["lonC"], cells["latC"], lonP, latP, cells["areaCell"], phi_s)]
+    rc = L.mcv_history_init(os.fsencode(path), ctypes.byref(info), *[_ptr(a) for a in arrs])
+    if rc != MCV_OK:
+        raise MCVError(rc, L.mcv_hip_last_error().decode())
+
+
+def history_write(path, time_slot, rec, dof, n, lib=None):
+    """history_write_stat (output_mod.F90:168-223) of host arrays."""
+    L = load(lib)
+    f, keep = _history_struct(rec, dims(dof, n)[2], n + 2)
+    rc = L.mcv_history_write(os.fsencode(path), int(time_slot), ctypes.byref(f))
+    if rc != MCV_OK:
+        raise MCVError(rc, L.mcv_hip_last_error().decode())
+
+
 def footprint(dof, n, precision=FP64, scheme="RK4", device=0, lib=None):
     """Device bytes one solve of this configuration allocates at most."""
     L = load(lib)
@@ -222,6 +282,24 @@ class Solver:
         m = np.empty((6, self.nh, self.nh))
         self._check(self.lib.mcv_hip_sphere_winds(self._h, _ptr(z), _ptr(m)))
         return z, m
+
+    def cell_averages(self):
+        """calc_VIA of u, v, phi on the device: (uC, vC, phiC), each (6, n+2, n+2)."""
+        out = [np.empty((6, self.n + 2, self.n + 2)) for _ in range(3)]
+        self._check(self.lib.mcv_hip_cell_averages(self._h, *[_ptr(o) for o in out]))
+        return tuple(out)
+
+    def history_fields(self):
+        """The record fields of the output block after a step (dict)."""
+        rec = {k: np.empty((6, self.nh, self.nh)) for k in HISTORY_POINT_FIELDS}
+        rec.update({k: np.empty((6, self.n + 2, self.n + 2)) for k in HISTORY_CELL_FIELDS})
+        f, _ = _history_struct(rec, self.nh, self.n + 2)
+        self._check(self.lib.mcv_hip_history_fields(self._h, ctypes.byref(f)))
+        return rec
+
+    def history_write(self, path, time_slot):
+        """The output block of MCV_SW.F90:62-70: record `time_slot` of `path`."""
+        self._check(self.lib.mcv_hip_history_write(self._h, os.fsencode(path), int(time_slot)))
 
     def tendency(self, state):
         keep = [np.ascontiguousarray(state[k], dtype=np.float64) for k in STATE_FIELDS]

@@ -79,6 +79,10 @@ struct mcv_handle {
   int diag_blocks = 0;
   double *diag_out = nullptr;
   double *pack[6] = {};
+  // history fields (allocated by the first mcv_hip_history_fields call):
+  // u, v, contraU, contraV, phi, phi_t, zonal, meridional; uC, vC, phiC
+  double *hist[8] = {};
+  double *histc[3] = {};
   int64_t steps = 0;
   int tile = 0;  // index into the tile shapes of the solve's order
   // per-kernel timing (mcv_hip_step_timed)

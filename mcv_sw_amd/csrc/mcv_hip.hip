@@ -615,6 +615,61 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   return MCV_OK;
 }
 
+// History fields on the device (lazily allocated scratch, kept by the handle).
+int history_buffers(mcv_handle *h, bool points) {
+  int rc;
+  for (int f = 0; f < 8 && points; ++f)
+    if (!h->hist[f] && (rc = dev_alloc(h, &h->hist[f], h->np))) return rc;
+  const size_t nc = 6 * (size_t)(h->cfg.n + 2) * (h->cfg.n + 2);
+  for (int f = 0; f < 3; ++f)
+    if (!h->histc[f] && (rc = dev_alloc(h, &h->histc[f], nc))) return rc;
+  return MCV_OK;
+}
+
+int launch_cell_averages(mcv_handle *h, const double *a, const double *b, const double *c) {
+  const long long nc = 6LL * (h->cfg.n + 2) * (h->cfg.n + 2);
+  const dim3 grid((unsigned)std::min<long long>((nc + 255) / 256, 4096)), block(256);
+  if (h->cfg.dof == 3)
+    hipLaunchKernelGGL(cell_average_kernel<3>, grid, block, 0, h->stream, h->mesh, h->cfg.n, a, b,
+                       c, h->histc[0], h->histc[1], h->histc[2]);
+  else
+    hipLaunchKernelGGL(cell_average_kernel<4>, grid, block, 0, h->stream, h->mesh, h->cfg.n, a, b,
+                       c, h->histc[0], h->histc[1], h->histc[2]);
+  HIP_TRY(hipGetLastError());
+  return MCV_OK;
+}
+
+// The output block of MCV_SW.F90:62-70 on the current state: hist[0..7] and
+// histc[0..2] hold the record's fields.
+int history_fields_dev(mcv_handle *h) {
+  if (h->steps == 0)
+    return fail(MCV_ESTATE,
+                "no time step has run: the first history record is written from the host's "
+                "initial arrays (MCV_SW.F90:50)");
+  int rc = history_buffers(h, true);
+  if (rc) return rc;
+  const double *y[3];
+  if ((rc = materialize(h, y))) return rc;
+  hipLaunchKernelGGL(history_point_kernel, dim3(2048), dim3(256), 0, h->stream, h->mesh, y[0],
+                     y[1], y[2], h->hist[0], h->hist[1], h->hist[2], h->hist[3], h->hist[4],
+                     h->hist[5], h->hist[6], h->hist[7]);
+  HIP_TRY(hipGetLastError());
+  return launch_cell_averages(h, h->hist[0], h->hist[1], h->hist[4]);
+}
+
+int download_history(mcv_handle *h, const mcv_history_fields *out) {
+  double *dst[11] = {out->u, out->v, out->contraU, out->contraV, out->phi, out->phi_t,
+                     out->zonal, out->meridional, out->uC, out->vC, out->phiC};
+  const size_t nc = 6 * (size_t)(h->cfg.n + 2) * (h->cfg.n + 2);
+  for (int f = 0; f < 11; ++f)
+    if (dst[f])
+      HIP_TRY(hipMemcpyAsync(dst[f], f < 8 ? h->hist[f] : h->histc[f - 8],
+                             (f < 8 ? h->np : nc) * sizeof(double), hipMemcpyDeviceToHost,
+                             h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return MCV_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -690,6 +745,56 @@ int mcv_hip_sphere_winds(mcv_handle *h, double *zonal, double *meridional) {
                          h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return MCV_OK;
+}
+
+int mcv_hip_cell_averages(mcv_handle *h, double *uC, double *vC, double *phiC) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (!uC || !vC || !phiC) return fail(MCV_EINVAL, "output array is NULL");
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  int rc = history_buffers(h, false);
+  if (rc) return rc;
+  const double *y[3];
+  if ((rc = materialize(h, y))) return rc;
+  if ((rc = launch_cell_averages(h, y[1], y[2], y[0]))) return rc;
+  mcv_history_fields f{};
+  f.uC = uC;
+  f.vC = vC;
+  f.phiC = phiC;
+  return download_history(h, &f);
+}
+
+int mcv_hip_history_fields(mcv_handle *h, const mcv_history_fields *out) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (!out) return fail(MCV_EINVAL, "out is NULL");
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  int rc = history_fields_dev(h);
+  if (rc) return rc;
+  return download_history(h, out);
+}
+
+int mcv_hip_history_write(mcv_handle *h, const char *path, int32_t time_slot) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (!path) return fail(MCV_EINVAL, "path is NULL");
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  int rc = history_fields_dev(h);
+  if (rc) return rc;
+  const size_t nc = 6 * (size_t)(h->cfg.n + 2) * (h->cfg.n + 2);
+  std::vector<double> buf;
+  try {
+    buf.resize(8 * (size_t)h->np + 3 * nc);
+  } catch (const std::bad_alloc &) {
+    return fail(MCV_ENOMEM, "host staging for the history record failed");
+  }
+  mcv_history_fields f{};
+  double **slot[11] = {&f.u, &f.v, &f.contraU, &f.contraV, &f.phi, &f.phi_t,
+                       &f.zonal, &f.meridional, &f.uC, &f.vC, &f.phiC};
+  size_t off = 0;
+  for (int k = 0; k < 11; ++k) {
+    *slot[k] = buf.data() + off;
+    off += k < 8 ? (size_t)h->np : nc;
+  }
+  if ((rc = download_history(h, &f))) return rc;
+  return mcv_history_write(path, time_slot, &f);
 }
 
 int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG, double *t_u,

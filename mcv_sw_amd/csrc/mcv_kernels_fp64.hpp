@@ -173,6 +173,82 @@ static __global__ void sphere_winds_kernel(MeshDev m, const double *u, const dou
   }
 }
 
+// FillValue of constants_mod.F90:20.
+constexpr double kFillValue = -9999999999999999.0;
+
+// Point fields of one history record, as the output block of MCV_SW.F90:62-70
+// leaves them in stat(new) after a time step:
+//   addFillValue (output_mod.F90:230-248) sets u, v on the four halo corners
+//   to FillValue; fill_ghost (ghost_mod.f90:160-179) left phi there at
+//   FillValue; convert_wind_P2SP (spatial_operators_mod.F90:391-405) then
+//   forms the sphere winds on every point, corners included; contraU/V are
+//   cov2contrav (projection_mod.F90:138-148) of the same u, v, as the last
+//   boundary pass formed them; phi_t = phi + phi_s (output_mod.F90:207).
+static __global__ void history_point_kernel(MeshDev m, const double *phi, const double *u,
+                                            const double *v, double *o_u, double *o_v,
+                                            double *o_cu, double *o_cv, double *o_phi,
+                                            double *o_phit, double *o_zon, double *o_mer) {
+  const int H = m.dof - 1, npv = m.npv, nh = m.nh;
+  const long long per = (long long)nh * nh, np = 6 * per;
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
+       g += (long long)gridDim.x * blockDim.x) {
+    const long long p = g / per, off = g - p * per;
+    const int i = (int)(off % nh) + 1 - H, j = (int)(off / nh) + 1 - H;
+    const bool corner = (i < 1 || i > npv) && (j < 1 || j > npv);
+    const long long mi = p * m.metric_pstride + off;
+    const double uu = corner ? kFillValue : u[g];
+    const double vv = corner ? kFillValue : v[g];
+    const double ph = corner ? kFillValue : phi[g];
+    o_u[g] = uu;
+    o_v[g] = vv;
+    o_cu[g] = m.ig11[mi] * uu + m.ig12[mi] * vv;
+    o_cv[g] = m.ig12[mi] * uu + m.ig22[mi] * vv;
+    o_phi[g] = ph;
+    o_phit[g] = ph + m.phi_s[p * m.phis_pstride + off];
+    double z, w;
+    apply2(m.p2sp + 4 * g, uu, vv, z, w);
+    o_zon[g] = z;
+    o_mer[g] = w;
+  }
+}
+
+// calc_VIA (diag_mod.F90:69-93) of three point fields: cell (i, j) of panel p
+// is the weighted sum of its DOF x DOF points (pvIdx, mesh_mod.F90:63-73),
+// accumulated from 0 with iDOF fastest, the weights those of
+// mesh_mod.F90:183-203 ([1,4,1] x [1,4,1] / 36, [1,3,3,1] x [1,3,3,1] / 80).
+// Cells are (n + 2)^2 per panel, the halo ring (ics = 0, ice = n + 1) zero.
+template <int DOF>
+__global__ void cell_average_kernel(MeshDev m, int n, const double *a, const double *b,
+                                    const double *c, double *oa, double *ob, double *oc) {
+  constexpr int H = DOF - 1;
+  constexpr double kRow[4] = {1.0, DOF == 3 ? 4.0 : 3.0, DOF == 3 ? 1.0 : 3.0, 1.0};
+  constexpr double kDen = DOF == 3 ? 36.0 : 80.0;
+  const int nc = n + 2, nh = m.nh;
+  const long long per = (long long)nc * nc, ncell = 6 * per;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < ncell;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long p = t / per, off = t - p * per;
+    const int ic = (int)(off % nc), jc = (int)(off / nc);  // 0 .. n+1
+    double sa = 0.0, sb = 0.0, sc = 0.0;
+    if (ic >= 1 && ic <= n && jc >= 1 && jc <= n) {
+#pragma unroll
+      for (int jd = 0; jd < DOF; ++jd)
+#pragma unroll
+        for (int id = 0; id < DOF; ++id) {
+          const double w = (kRow[id] * kRow[jd]) / kDen;
+          const int ip = (ic - 1) * H + id + 1, jp = (jc - 1) * H + jd + 1;
+          const long long g = pidx(nh, H, (int)p + 1, ip, jp);
+          sa = sa + w * a[g];
+          sb = sb + w * b[g];
+          sc = sc + w * c[g];
+        }
+    }
+    oa[t] = sa;
+    ob[t] = sb;
+    oc[t] = sc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // FP64 integrals (diag_mod.F90:10-67): per-cell sums in the reference order,
 // then a deterministic double-double tree over cells and blocks.

@@ -22,7 +22,8 @@ module mcv_host_lib
   integer, parameter :: qp = selected_real_kind(33)
   real(dp), parameter :: pi = 2.0_dp * asin(1.0_dp)
 
-  public :: mcv_host_mesh, mcv_host_ghost, mcv_host_case, mcv_host_jet, mcv_host_handoff
+  public :: mcv_host_mesh, mcv_host_ghost, mcv_host_case, mcv_host_jet, mcv_host_handoff, &
+            mcv_host_cells
 
   ! State of L'Ecuyer's MRG32k3a combined multiple-recursive generator
   ! (Operations Research 47(1), 1999), kept in integer(8) with no overflow.
@@ -171,6 +172,67 @@ contains
     call coriolis(k, omega, latP, f)
     mcv_host_mesh = 0
   end function mcv_host_mesh
+
+  ! Cell geometry of the history file (mesh_mod.F90:165-214): cell-centre
+  ! longitude and latitude of cells 0..n+1 (one halo cell), centre angles
+  ! halfway between the cell's first and last point angles; and the cell
+  ! areas, zero on the halo ring.  An equiangular cell is a spherical
+  ! quadrilateral whose area is its spherical excess, the sum of its four
+  ! interior angles minus 2 pi; at the grid vertex (a, b) the grid lines meet
+  ! at acos(-sin a sin b), and the interior angles at the cell's (+1, 0) and
+  ! (0, +1) vertices are the supplements.  The vertex angles are formed in
+  ! REAL(16) and rounded once, as the reference's dble() is under
+  ! -fdefault-real-8.  Arrays (n+2, n+2, 6).
+  integer(c_int) function mcv_host_cells(dof, n, radius, lonC, latC, areaCell) &
+      bind(C, name='mcv_host_cells')
+    integer(c_int), value :: dof, n
+    real(c_double), value :: radius
+    real(c_double), intent(out) :: lonC(0:n+1, 0:n+1, 6), latC(0:n+1, 0:n+1, 6)
+    real(c_double), intent(out) :: areaCell(0:n+1, 0:n+1, 6)
+    integer :: H, p, ic, jc, k, k1, k2
+    real(dp) :: d2r, dx, xmin, xs, xe, ys, ye, xc, yc
+    real(dp), allocatable :: gp(:), ang(:, :), da(:, :)
+    mcv_host_cells = -1
+    if ((dof /= 3 .and. dof /= 4) .or. n < 2) return
+    H = dof - 1
+    d2r = pi / 180.
+    dx = (90.0_dp / real(n, dp)) * d2r
+    xmin = -45.0_dp * d2r
+    do p = 1, 6
+      do jc = 0, n + 1
+        do ic = 0, n + 1
+          xs = ((ic - 1) * H) * dx / (dof - 1) + xmin
+          xe = (ic * H) * dx / (dof - 1) + xmin
+          ys = ((jc - 1) * H) * dx / (dof - 1) + xmin
+          ye = (jc * H) * dx / (dof - 1) + xmin
+          xc = 0.5 * (xs + xe)
+          yc = 0.5 * (ys + ye)
+          call panel_to_sphere(xc, yc, p, lonC(ic, jc, p), latC(ic, jc, p))
+        enddo
+      enddo
+    enddo
+    allocate(gp(n + 1), ang(n + 1, n + 1), da(n, n))
+    do k = 1, n + 1
+      gp(k) = real(real(-0.25_dp * pi, qp) + (real(pi, qp) / real(2 * n, qp)) * real(k - 1, qp), dp)
+    enddo
+    do k2 = 1, n + 1
+      do k1 = 1, n + 1
+        ang(k1, k2) = acos(-sin(gp(k1)) * sin(gp(k2)))
+      enddo
+    enddo
+    do k2 = 1, n
+      do k1 = 1, n
+        da(k1, k2) = -2.0_dp * pi + ang(k1, k2) + (pi - ang(k1 + 1, k2)) + &
+                     (pi - ang(k1, k2 + 1)) + ang(k1 + 1, k2 + 1)
+      enddo
+    enddo
+    da = da * (radius * radius)
+    areaCell = 0.0_dp
+    do p = 1, 6
+      areaCell(1:n, 1:n, p) = da
+    enddo
+    mcv_host_cells = 0
+  end function mcv_host_cells
 
   ! f = 2 Omega sin(lat) (mesh_mod.F90:160), in a pass of its own.
   subroutine coriolis(np, omega, lat, f)

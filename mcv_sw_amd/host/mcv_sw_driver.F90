@@ -10,8 +10,12 @@
 !                   jet of &jet_settings
 !   &domain         dx (degrees; 90/dx cells per panel edge), dy = dx
 !   &gpu_settings   dof (3 = MCV3, 4 = MCV4; the reference's -DMCV3/-DMCV4),
-!                   precision ('FP64' or 'FP32C'), device, dump_file
+!                   precision ('FP64' or 'FP32C'), device, dump_file,
+!                   history_file (default 'mcv_output.nc', the reference's
+!                   ncFile of output_mod.F90:10; '' writes no history)
 !   &jet_settings   u0, h0, lmax, seed, rms_h
+! History: the reference's netCDF file and cadence (MCV_SW.F90:49-50,62-70),
+! the first record from the initial arrays, later ones from the device.
 ! The final state (phi, u, v on (nh, nh, 6), stream access) is written to
 ! dump_file when it is set.
 module mcv_hip_binding
@@ -35,6 +39,15 @@ module mcv_hip_binding
     type(c_ptr) :: phi, phiG, u, v, contraU, contraV
   end type mcv_state
 
+  type, bind(C) :: mcv_history_info
+    integer(c_int32_t) :: dof, n, xhalo, yhalo, case_num
+    real(c_double) :: dx, dy, dt
+  end type mcv_history_info
+
+  type, bind(C) :: mcv_history_fields
+    type(c_ptr) :: u, v, contraU, contraV, phi, phi_t, zonal, meridional, uC, vC, phiC
+  end type mcv_history_fields
+
   interface
     integer(c_int) function mcv_hip_init(cfg, mesh, initial, h) bind(C, name='mcv_hip_init')
       import :: c_int, c_ptr, mcv_config, mcv_mesh, mcv_state
@@ -51,11 +64,35 @@ module mcv_hip_binding
     end function mcv_hip_step
 
     integer(c_int) function mcv_hip_sync(h, out, mass, energy) bind(C, name='mcv_hip_sync')
-      import :: c_int, c_ptr, c_double, mcv_state
+      import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
-      type(mcv_state), intent(in) :: out
+      type(c_ptr), value :: out  ! c_loc of an mcv_state, or c_null_ptr
       real(c_double), intent(out) :: mass, energy
     end function mcv_hip_sync
+
+    integer(c_int) function mcv_history_init(path, info, lonC, latC, lonP, latP, areaCell, &
+                                             phi_s) bind(C, name='mcv_history_init')
+      import :: c_int, c_char, c_double, mcv_history_info
+      character(kind=c_char), intent(in) :: path(*)
+      type(mcv_history_info), intent(in) :: info
+      real(c_double), intent(in) :: lonC(*), latC(*), lonP(*), latP(*), areaCell(*), phi_s(*)
+    end function mcv_history_init
+
+    integer(c_int) function mcv_history_write(path, time_slot, rec) &
+        bind(C, name='mcv_history_write')
+      import :: c_int, c_int32_t, c_char, mcv_history_fields
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: time_slot
+      type(mcv_history_fields), intent(in) :: rec
+    end function mcv_history_write
+
+    integer(c_int) function mcv_hip_history_write(h, path, time_slot) &
+        bind(C, name='mcv_hip_history_write')
+      import :: c_int, c_int32_t, c_char, c_ptr
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: time_slot
+    end function mcv_hip_history_write
 
     integer(c_int) function mcv_hip_free(h) bind(C, name='mcv_hip_free')
       import :: c_int, c_ptr
@@ -107,7 +144,7 @@ program mcv_sw_gpu
   integer :: case_num = 2
   integer :: dof = 3, device = 0
   character(len=16) :: precision = 'FP64'
-  character(len=256) :: dump_file = ''
+  character(len=256) :: dump_file = '', history_file = 'mcv_output.nc'
   real(c_double) :: u0 = 20.0_c_double, h0 = 5960.0_c_double, rms_h = 30.0_c_double
   integer :: lmax = 8
   integer(c_int64_t) :: seed = 0
@@ -115,7 +152,7 @@ program mcv_sw_gpu
                            history_interval, integral_scheme
   namelist /case_select/ case_num
   namelist /domain/ dx, dy, xhalo, yhalo
-  namelist /gpu_settings/ dof, precision, device, dump_file
+  namelist /gpu_settings/ dof, precision, device, dump_file, history_file
   namelist /jet_settings/ u0, h0, lmax, seed, rms_h
 
   integer :: n, H, npv, nh, np, ios, it, nsteps, pending, output_idx, total_output_num
@@ -127,11 +164,15 @@ program mcv_sw_gpu
   integer(c_int32_t), allocatable, target :: iref(:, :)
   real(c_double), allocatable, target :: phi(:), zonal(:), merid(:), phiG(:), u(:), v(:)
   real(c_double), allocatable, target :: cu(:), cv(:)
+  real(c_double), allocatable, target :: lonC(:), latC(:), areaCell(:), phi_t(:), zero_c(:)
   type(mcv_config) :: cfg
   type(mcv_mesh) :: mesh
-  type(mcv_state) :: st
+  type(mcv_state), target :: st
+  type(mcv_history_info) :: hinfo
+  type(mcv_history_fields) :: hrec
   type(c_ptr) :: solve
-  real(c_double) :: mass0, energy0, mass, energy
+  real(c_double) :: mass0, energy0, mass, energy, mass_old, energy_old
+  logical :: history
   integer(8) :: t_start, t_end, t_rate
 
   call system_clock(t_start, t_rate)
@@ -201,23 +242,48 @@ program mcv_sw_gpu
   ! history cadence of MCV_SW.F90:48-77
   output_idx = 1
   total_output_num = int(nsteps * dt / history_interval)
-  call mcv_check(mcv_hip_sync(solve, st, mass0, energy0), 'mcv_hip_sync')
+  history = len_trim(history_file) > 0
+  if (history) then
+    ! history_init + history_write_stat(stat(old), 1) (MCV_SW.F90:49-50); the
+    ! reference's cell averages are not formed before the first record
+    allocate(lonC((n + 2) * (n + 2) * 6), latC((n + 2) * (n + 2) * 6), &
+             areaCell((n + 2) * (n + 2) * 6), zero_c((n + 2) * (n + 2) * 6), phi_t(np))
+    if (mcv_host_cells(dof, n, radius, lonC, latC, areaCell) /= 0) stop 'cell setup failed'
+    hinfo = mcv_history_info(dof, n, xhalo, yhalo, case_num, dx * d2r, dy * d2r, dt)
+    call mcv_check(mcv_history_init(trim(history_file)//c_null_char, hinfo, lonC, latC, lonP, &
+                                    latP, areaCell, phi_s), 'history_init')
+    phi_t = phi + phi_s
+    zero_c = 0.0_c_double
+    hrec = mcv_history_fields(c_loc(u), c_loc(v), c_loc(cu), c_loc(cv), c_loc(phi), &
+                              c_loc(phi_t), c_loc(zonal), c_loc(merid), c_loc(zero_c), &
+                              c_loc(zero_c), c_loc(zero_c))
+    call mcv_check(mcv_history_write(trim(history_file)//c_null_char, 1, hrec), &
+                   'history_write_stat')
+  endif
+  call mcv_check(mcv_hip_sync(solve, c_null_ptr, mass0, energy0), 'mcv_hip_sync')
   print *, 'output index/total, MCR, ECR :', output_idx - 1, '/', total_output_num, ' ', 0., 0.
   pending = 0
   do it = 1, nsteps
     pending = pending + 1
     if (mod(it * dt, real(history_interval, c_double)) == 0. .and. &
         it * dt >= history_interval) then
-      call mcv_check(mcv_hip_step(solve, int(pending, c_int32_t)), 'mcv_hip_step')
+      ! the energy ratio is the one of stat(old), the state before this step
+      ! (MCV_SW.F90:72)
+      if (pending > 1) call mcv_check(mcv_hip_step(solve, int(pending - 1, c_int32_t)), &
+                                      'mcv_hip_step')
+      call mcv_check(mcv_hip_sync(solve, c_null_ptr, mass_old, energy_old), 'mcv_hip_sync')
+      call mcv_check(mcv_hip_step(solve, 1_c_int32_t), 'mcv_hip_step')
       pending = 0
       output_idx = output_idx + 1
-      call mcv_check(mcv_hip_sync(solve, st, mass, energy), 'mcv_hip_sync')
+      if (history) call mcv_check(mcv_hip_history_write(solve, trim(history_file)//c_null_char, &
+                                  int(output_idx, c_int32_t)), 'history_write_stat')
+      call mcv_check(mcv_hip_sync(solve, c_null_ptr, mass, energy), 'mcv_hip_sync')
       print *, 'output index/total, MCR, ECR :', output_idx - 1, '/', total_output_num, ' ', &
-               (mass - mass0) / mass0, (energy - energy0) / energy0
+               (mass - mass0) / mass0, (energy_old - energy0) / energy0
     endif
   enddo
   if (pending > 0) call mcv_check(mcv_hip_step(solve, int(pending, c_int32_t)), 'mcv_hip_step')
-  call mcv_check(mcv_hip_sync(solve, st, mass, energy), 'mcv_hip_sync')
+  call mcv_check(mcv_hip_sync(solve, c_loc(st), mass, energy), 'mcv_hip_sync')
   if (len_trim(dump_file) > 0) then
     open(2, file=trim(dump_file), access='stream', form='unformatted', status='replace')
     write(2) phi, u, v

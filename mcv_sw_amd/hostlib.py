@@ -37,8 +37,9 @@ def load():
         L.mcv_host_case.argtypes = [ci, ci, cd, cd, cd] + [_D] * 6
         L.mcv_host_jet.argtypes = [ci, cd, cd, cd, cd, cd, ci, ctypes.c_int64, cd] + [_D] * 6
         L.mcv_host_handoff.argtypes = [ci] + [_D] * 11
+        L.mcv_host_cells.argtypes = [ci, ci, cd, _D, _D, _D]
         for f in ("mcv_host_mesh", "mcv_host_ghost", "mcv_host_case", "mcv_host_jet",
-                  "mcv_host_handoff"):
+                  "mcv_host_handoff", "mcv_host_cells"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -70,6 +71,18 @@ def mesh(dof, n, radius=RADIUS, omega=OMEGA):
     m["dx"] = (90.0 / n) * (PI / 180.0)
     m["dof"], m["n"] = dof, n
     return m
+
+
+def cells(dof, n, radius=RADIUS):
+    """Cell geometry of the history file (mesh_mod.F90:165-214): lonC, latC
+    (radians) and areaCell (m^2), each (6, n+2, n+2), the halo ring of
+    areaCell zero."""
+    s = (6, n + 2, n + 2)
+    c = {k: np.empty(s) for k in ("lonC", "latC", "areaCell")}
+    rc = load().mcv_host_cells(dof, n, radius, _p(c["lonC"]), _p(c["latC"]), _p(c["areaCell"]))
+    if rc:
+        raise ValueError(f"mcv_host_cells(dof={dof}, n={n}) failed: {rc}")
+    return c
 
 
 def handoff(m, phi, zonal, merid):

@@ -17,8 +17,9 @@
 !   namelist.input  reference namelist (parameters_mod.F90:86-99)
 !   driver.nml      &driver nsteps, ic_file, out_dir, dump_steps, history_every,
 !                   dump_mesh, dump_states
-! Outputs (out_dir): mesh.bin, ghost.bin, state_NNNNNNN.bin (raw float64,
-! Fortran order, halo included), and one JSON timing line on stdout.
+! Outputs (out_dir): mesh.bin, ghost.bin, cells.bin, state_NNNNNNN.bin (raw
+! float64, Fortran order, halo included), via_NNNNNNN.bin (cell averages of
+! history steps), and one JSON timing line on stdout.
 module refdrv_scheme
   use parameters_mod
   use stat_mod
@@ -192,6 +193,10 @@ program ref_driver
     write(u) ghost%coef
     write(u) ghost%iref
     close(u)
+    ! cell geometry of the history file (mesh_mod.F90:165-214)
+    open(newunit=u, file=trim(out_dir)//'/cells.bin', access='stream', form='unformatted', status='replace')
+    write(u) mesh%lonC, mesh%latC, mesh%areaCell
+    close(u)
   endif
 
   if (dump_states) call dump_state(0, stat(0))
@@ -245,6 +250,15 @@ contains
     write(uu) s%phi, s%phiG, s%u, s%v, s%contraU, s%contraV, s%zonal_wind, s%meridional_wind
     write(uu) total_mass, total_energy
     close(uu)
+    ! cell averages of a history step (calc_VIA above)
+    if (history_every > 0 .and. step > 0) then
+      if (mod(step, history_every) == 0) then
+        write(fname,'(a,a,i7.7,a)') trim(out_dir), '/via_', step, '.bin'
+        open(newunit=uu, file=trim(fname), access='stream', form='unformatted', status='replace')
+        write(uu) s%uC, s%vC, s%phiC
+        close(uu)
+      endif
+    endif
   end subroutine dump_state
 
 end program ref_driver

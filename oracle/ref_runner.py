@@ -102,6 +102,9 @@ def run(dof, n, dt, nsteps, scheme="RK4", case_num=2, ic=None, dump_steps=(),
             g = os.path.join(wd, "ghost.bin")
             m["coef"] = np.fromfile(g, dtype="<f8", count=npv * h).reshape(h, npv)
             m["iref"] = np.fromfile(g, dtype="<i4", offset=npv * h * 8).reshape(h, npv)
+            c = np.fromfile(os.path.join(wd, "cells.bin"), dtype="<f8").reshape(
+                (3, 6, n + 2, n + 2))
+            m["lonC"], m["latC"], m["areaCell"] = c[0], c[1], c[2]
             out["mesh"] = m
         for fn in sorted(os.listdir(wd)):
             if fn.startswith("state_") and fn.endswith(".bin"):
@@ -111,6 +114,10 @@ def run(dof, n, dt, nsteps, scheme="RK4", case_num=2, ic=None, dump_steps=(),
                 st = dict(zip(("phi", "phiG", "u", "v", "contraU", "contraV", "zonal", "merid"),
                               flds))
                 st["mass"], st["energy"] = float(a[-2]), float(a[-1])
+                vf = os.path.join(wd, f"via_{step:07d}.bin")
+                if os.path.exists(vf):
+                    c = np.fromfile(vf, dtype="<f8").reshape((3, 6, n + 2, n + 2))
+                    st["uC"], st["vC"], st["phiC"] = c[0], c[1], c[2]
                 out["states"][step] = st
         return out
     finally:

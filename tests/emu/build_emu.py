@@ -13,7 +13,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
-SRCS = [os.path.join(ROOT, "mcv_sw_amd", "csrc", f) for f in ("mcv_hip.hip", "mcv_batch.hip", "mcv_probe.hip")]
+SRCS = [os.path.join(ROOT, "mcv_sw_amd", "csrc", f) for f in ("mcv_hip.hip", "mcv_batch.hip", "mcv_probe.hip",
+                                                                 "mcv_netcdf.cpp")]
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libmcv_hip_emu.so")
 ASAN = os.path.join(OUT, "asan_driver")
@@ -31,7 +32,7 @@ def _stale(target, deps):
 
 def _deps():
     d = os.path.join(ROOT, "mcv_sw_amd", "csrc")
-    out = [os.path.join(d, f) for f in os.listdir(d) if f.endswith((".hip", ".hpp"))]
+    out = [os.path.join(d, f) for f in os.listdir(d) if f.endswith((".hip", ".hpp", ".cpp"))]
     return out + [os.path.join(HERE, "hip", "hip_runtime.h"), os.path.join(HERE, "asan_driver.cpp"),
                   os.path.join(ROOT, "include", "mcv_hip.h")]
 

@@ -10,6 +10,11 @@ compiled unmodified; see oracle/Makefile) in this container and records:
   tiny_*.npz    full step-0 and final arrays of the 4x4-cell cases
   *_full_ghost.npz  the reference's ghost tables (coef, iref) of the
                 full-size case
+  history_*.npz the history records of small runs (the arguments of
+                history_write_stat after the output block of MCV_SW.F90:62-70:
+                addFillValue, convert_wind_P2SP, calc_VIA) with their initial
+                state and cell geometry; golden.json "cells" holds digests of
+                the reference's lonC, latC, areaCell per mesh
 Run: python tests/golden/make_golden.py [case ...]   (needs `make -C oracle ref`)
 """
 import hashlib
@@ -69,6 +74,31 @@ CASES = [
 ]
 
 MESHES = [(3, 4), (4, 4), (3, 8), (3, 32), (4, 12), (4, 16), (4, 48), (3, 90)]
+CELL_KEYS = ("lonC", "latC", "areaCell")
+
+# (name, dof, n, case_num, scheme, dt, steps, history_every)
+HISTORY = [
+    ("history_mcv3_n4_case5_rk4", 3, 4, 5, "RK4", 1200.0, 6, 3),
+    ("history_mcv4_n4_case6_rk3", 4, 4, 6, "RK3_TVD", 900.0, 6, 3),
+]
+
+
+def write_history(name, dof, n, case_num, scheme, dt, steps, every):
+    r = ref_runner.run(dof, n, dt, steps, scheme=scheme, case_num=case_num, history_every=every,
+                       dump_steps=tuple(range(every, steps + 1, every)))
+    arrs = {"phi_s": r["mesh"]["phi_s"]}
+    for k in CELL_KEYS:
+        arrs[k] = r["mesh"][k]
+    s0 = r["states"][0]
+    for k in FIELDS + ("zonal", "merid"):
+        arrs["s0_" + k] = s0[k]
+    for step in range(every, steps + 1, every):
+        st = r["states"][step]
+        for k in ("u", "v", "contraU", "contraV", "phi", "zonal", "merid", "uC", "vC", "phiC"):
+            arrs[f"r{step}_{k}"] = st[k]
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrs)
+    return {"dof": dof, "n": n, "case_num": case_num, "scheme": scheme, "dt": dt,
+            "steps": steps, "history_every": every}
 
 
 def main(only=None):
@@ -87,6 +117,17 @@ def main(only=None):
         r = ref_runner.run(dof, n, dt=60.0, nsteps=0, case_num=2)
         out["meshes"][f"{dof}_{n}"] = {k: digest(r["mesh"][k]) for k in MESH_KEYS}
         print("mesh", dof, n, flush=True)
+    if not only or "cells" in only:
+        out["cells"] = {}
+        for dof, n in MESHES:
+            r = ref_runner.run(dof, n, dt=60.0, nsteps=0, case_num=2)
+            out["cells"][f"{dof}_{n}"] = {k: digest(r["mesh"][k]) for k in CELL_KEYS}
+            print("cells", dof, n, flush=True)
+    out.setdefault("history", {})
+    for h in HISTORY:
+        if not only or h[0] in only:
+            out["history"][h[0]] = write_history(*h)
+            print("history", h[0], flush=True)
     for name, dof, n, init, scheme, dt, steps, extra in run_cases:
         ic = None
         case_num = 2

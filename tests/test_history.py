@@ -1,0 +1,391 @@
+"""History output and cell-average diagnostics against the reference.
+
+Pins (tests/golden, written by make_golden.py from the reference build):
+  - cell geometry (lonC, latC, areaCell of mesh_mod.F90:165-214) of the host
+    library, bit for bit, on every golden mesh;
+  - the record fields of the output block of MCV_SW.F90:62-70 (addFillValue,
+    convert_wind_P2SP, calc_VIA, phi + phi_s) after a few steps, bit for bit,
+    halo corners included: through the HIP sources compiled for the CPU and,
+    on the GPU, through the HIP library;
+  - the file itself: the library's netCDF writer read back with scipy's
+    independent reader of the classic format (dimensions, variables,
+    attributes, records) against the schema of output_mod.F90:37-166.
+
+One documented difference: at the first output after the start, the
+reference's contraU/contraV on the four halo corners come from u, v corner
+values its time level has never been given (uninitialised memory); those 4
+corner blocks are excluded at that record only.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mcv_sw_amd import capi, hostlib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+FILL = -9999999999999999.0
+R2D = 180.0 / (2.0 * np.arcsin(1.0))
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def corner_mask(dof, n):
+    h = dof - 1
+    npv = n * h + 1
+    nh = npv + 2 * h
+    m = np.zeros((nh, nh), bool)
+    for a in (slice(0, h), slice(h + npv, nh)):
+        for b in (slice(0, h), slice(h + npv, nh)):
+            m[a, b] = True
+    return m
+
+
+@pytest.mark.parametrize("key", sorted(GOLD["cells"]))
+def test_hostlib_cells_match_reference(key):
+    dof, n = map(int, key.split("_"))
+    c = hostlib.cells(dof, n)
+    assert {k: digest(c[k]) for k in ("lonC", "latC", "areaCell")} == GOLD["cells"][key]
+
+
+def test_cells_total_area_is_sphere():
+    """The cell areas are spherical excesses: they tile the sphere."""
+    for dof, n in ((3, 8), (4, 48)):
+        a = hostlib.cells(dof, n)["areaCell"]
+        assert a[:, 0, :].max() == 0 and a[:, -1, :].max() == 0
+        assert abs(a.sum() / (4 * np.pi * hostlib.RADIUS ** 2) - 1) < 1e-12
+
+
+def _history_case(name):
+    rec = GOLD["history"][name]
+    z = np.load(os.path.join(HERE, "golden", name + ".npz"))
+    m = hostlib.mesh(rec["dof"], rec["n"])
+    m["phi_s"] = z["phi_s"]
+    st = {k: z["s0_" + k] for k in capi.STATE_FIELDS}
+    return rec, z, m, st
+
+
+def _expected(z, step):
+    e = {k: z[f"r{step}_{k}"] for k in ("u", "v", "contraU", "contraV", "phi", "uC", "vC",
+                                        "phiC")}
+    e["zonal"], e["meridional"] = z[f"r{step}_zonal"], z[f"r{step}_merid"]
+    e["phi_t"] = e["phi"] + z["phi_s"]  # output_mod.F90:207
+    return e
+
+
+def _check_records(lib, name):
+    rec, z, m, st = _history_case(name)
+    dof, n, every = rec["dof"], rec["n"], rec["history_every"]
+    cm = corner_mask(dof, n)
+    with capi.Solver(dof, n, rec["dt"], m, st, scheme=rec["scheme"], lib=lib) as s:
+        with pytest.raises(capi.MCVError) as ei:
+            s.history_fields()  # the first record comes from the host's arrays
+        assert ei.value.code == capi.MCV_ESTATE
+        for step in range(every, rec["steps"] + 1, every):
+            s.step(every)
+            got, want = s.history_fields(), _expected(z, step)
+            for k, w in want.items():
+                g = got[k]
+                if k in ("contraU", "contraV") and step == every:
+                    g, w = g[:, ~cm], w[:, ~cm]
+                assert g.tobytes() == w.tobytes(), (name, step, k)
+            assert np.all(got["u"][:, cm] == FILL) and np.all(got["phi"][:, cm] == FILL)
+            uC, vC, phiC = s.cell_averages()
+            for k, a in (("uC", uC), ("vC", vC), ("phiC", phiC)):
+                assert a.tobytes() == want[k].tobytes(), (name, step, k)
+
+
+@pytest.mark.parametrize("name", sorted(GOLD["history"]))
+def test_emulated_history_fields_match_reference(name, emu_lib):
+    _check_records(emu_lib, name)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(GOLD["history"]))
+def test_history_fields_match_reference(name, hip_lib):
+    _check_records(hip_lib, name)
+
+
+def _read(path):
+    from scipy.io import netcdf_file
+    with netcdf_file(path, "r", mmap=False) as f:
+        out = {"version": f.version_byte, "dims": dict(f.dimensions),
+               "attrs": dict(f._attributes), "vars": {}}
+        for k, v in f.variables.items():
+            out["vars"][k] = {"dims": v.dimensions, "type": v.data.dtype.str,
+                              "attrs": dict(v._attributes), "data": v.data.copy()}
+    return out
+
+
+# output_mod.F90:39-137: (name, dimensions in the file (slowest first), units,
+# long_name, has _FillValue)
+SCHEMA = [
+    ("lonC", ("nPatch", "latC", "lonC"), "degree_east", "longitude on sphere coordinate for Cells", 0),
+    ("latC", ("nPatch", "latC", "lonC"), "degree_north", "latitude on sphere coordinate for Cells", 0),
+    ("lonP", ("nPatch", "latP", "lonP"), "degree_east", "longitude on sphere coordinate for Points", 0),
+    ("latP", ("nPatch", "latP", "lonP"), "degree_north", "latitude on sphere coordinate for Points", 0),
+    ("areaCell", ("nPatch", "latC", "lonC"), "m^2", "area of cells", 0),
+    ("phiP_s", ("nPatch", "latP", "lonP"), "m^2/s^2", "surface height on points", 1),
+    ("time", ("time",), "seconds", "time", 0),
+    ("uP", ("time", "nPatch", "latP", "lonP"), "m/s", "covariant wind vector on x direcvtion", 1),
+    ("vP", ("time", "nPatch", "latP", "lonP"), "m/s", "covariant wind vector on y direcvtion", 1),
+    ("contraUP", ("time", "nPatch", "latP", "lonP"), "m/s",
+     "contravariant wind vector on x direcvtion", 1),
+    ("contraVP", ("time", "nPatch", "latP", "lonP"), "m/s",
+     "contravariant wind vector on y direcvtion", 1),
+    ("phiP", ("time", "nPatch", "latP", "lonP"), "m^2/s^2", "geopotential height on points", 1),
+    ("phiP_t", ("time", "nPatch", "latP", "lonP"), "m^2/s^2",
+     "total geopotential height on points", 1),
+    ("zonal_wind", ("time", "nPatch", "latP", "lonP"), "m/s", "zonal wind", 1),
+    ("meridional_wind", ("time", "nPatch", "latP", "lonP"), "m/s", "meridional wind", 1),
+    ("uC", ("time", "nPatch", "latC", "lonC"), "m/s", "covariant wind VIA on x direcvtion", 1),
+    ("vC", ("time", "nPatch", "latC", "lonC"), "m/s", "covariant wind VIA on y direcvtion", 1),
+    ("phiC", ("time", "nPatch", "latC", "lonC"), "m^2/s^2", "geopotential height VIA", 1),
+]
+RECORD = {"uP": "u", "vP": "v", "contraUP": "contraU", "contraVP": "contraV", "phiP": "phi",
+          "phiP_t": "phi_t", "zonal_wind": "zonal", "meridional_wind": "meridional", "uC": "uC",
+          "vC": "vC", "phiC": "phiC"}
+
+
+def _records(dof, n, k, rng):
+    nh = n * (dof - 1) + 1 + 2 * (dof - 1)
+    out = []
+    for _ in range(k):
+        r = {f: rng.standard_normal((6, nh, nh)) for f in capi.HISTORY_POINT_FIELDS}
+        r.update({f: rng.standard_normal((6, n + 2, n + 2)) for f in capi.HISTORY_CELL_FIELDS})
+        out.append(r)
+    return out
+
+
+@pytest.mark.parametrize("dof,n", [(3, 4), (4, 6)])
+def test_history_file_schema_and_records(tmp_path, dof, n):
+    """The file as scipy's netCDF classic reader sees it: the reference's
+    dimensions, variables, types, attributes; static data; records written
+    out of order land in their slots."""
+    lib = capi.load()  # host-only entry points: no device needed
+    m = hostlib.mesh(dof, n)
+    m["phi_s"] = np.random.default_rng(1).standard_normal(m["sqrtG"].shape)
+    c = hostlib.cells(dof, n)
+    path = str(tmp_path / "mcv_output.nc")
+    dt = 600.0
+    capi.history_init(path, dof, n, dt, c, m["lonP"], m["latP"], m["phi_s"], case_num=5, lib=lib)
+    recs = _records(dof, n, 3, np.random.default_rng(7))
+    for slot in (1, 3, 2):
+        capi.history_write(path, slot, recs[slot - 1], dof, n, lib=lib)
+    f = _read(path)
+    h = dof - 1
+    npv = n * h + 1
+    assert f["version"] == 2  # NF90_64BIT_OFFSET
+    assert f["dims"] == {"lonC": n + 2, "latC": n + 2, "lonP": npv + 2 * h, "latP": npv + 2 * h,
+                         "nPatch": 6, "time": None}
+    dx = (np.pi / 2) / n
+    want_attrs = {"MCV_ORDER": dof, "dx": dx * R2D, "dy": dx * R2D, "dt": dt, "xhalo": 1,
+                  "yhalo": 1, "case_num": 5, "ics": 0, "ice": n + 1, "jcs": 0, "jce": n + 1,
+                  "ips": 1 - h, "ipe": npv + h, "jps": 1 - h, "jpe": npv + h, "ids": 1,
+                  "ide": npv, "jds": 1, "jde": npv, "ifs": 1, "ife": 6}
+    assert list(f["attrs"]) == list(want_attrs)
+    for k, v in want_attrs.items():
+        assert np.asarray(f["attrs"][k]).item() == v, k
+        assert np.asarray(f["attrs"][k]).dtype.kind == ("f" if k in ("dx", "dy", "dt") else "i")
+    assert list(f["vars"]) == [s[0] for s in SCHEMA]
+    for name, dims, units, long_name, fill in SCHEMA:
+        v = f["vars"][name]
+        assert v["dims"] == dims, name
+        assert v["type"] == (">i4" if name == "time" else ">f8"), name
+        at = v["attrs"]
+        assert list(at) == ["units", "long_name"] + (["_FillValue"] if fill else []), name
+        assert at["units"] == units.encode() and at["long_name"] == long_name.encode(), name
+        if fill:
+            assert np.asarray(at["_FillValue"]).item() == FILL
+    V = f["vars"]
+    assert np.array_equal(V["time"]["data"], [1, 2, 3])
+    assert V["lonC"]["data"].tobytes() == (c["lonC"] * R2D).astype(">f8").tobytes()
+    assert V["latP"]["data"].tobytes() == (m["latP"] * R2D).astype(">f8").tobytes()
+    assert V["areaCell"]["data"].tobytes() == c["areaCell"].astype(">f8").tobytes()
+    assert V["phiP_s"]["data"].tobytes() == m["phi_s"].astype(">f8").tobytes()
+    for var, key in RECORD.items():
+        for slot in (1, 2, 3):
+            assert np.array_equal(V[var]["data"][slot - 1], recs[slot - 1][key]), (var, slot)
+
+
+def test_history_write_rejects_bad_input(tmp_path):
+    lib = capi.load()
+    path = str(tmp_path / "not_netcdf.nc")
+    with open(path, "wb") as fh:
+        fh.write(b"HDF5 is not classic netCDF")
+    rec = _records(3, 4, 1, np.random.default_rng(0))[0]
+    with pytest.raises(capi.MCVError) as ei:
+        capi.history_write(path, 1, rec, 3, 4, lib=lib)
+    assert ei.value.code == capi.MCV_EINVAL
+    with pytest.raises(capi.MCVError):
+        capi.history_write(str(tmp_path / "missing.nc"), 1, rec, 3, 4, lib=lib)
+
+
+@pytest.mark.gpu
+def test_gpu_history_write_round_trip(tmp_path, hip_lib):
+    """mcv_hip_history_write produces the records mcv_hip_history_fields
+    returns, with the initial record written from the host's arrays."""
+    rec, z, m, st = _history_case("history_mcv3_n4_case5_rk4")
+    dof, n = rec["dof"], rec["n"]
+    c = hostlib.cells(dof, n)
+    path = str(tmp_path / "mcv_output.nc")
+    capi.history_init(path, dof, n, rec["dt"], c, m["lonP"], m["latP"], m["phi_s"],
+                      case_num=rec["case_num"], lib=hip_lib)
+    first = {"u": st["u"], "v": st["v"], "contraU": st["contraU"], "contraV": st["contraV"],
+             "phi": st["phi"], "phi_t": st["phi"] + m["phi_s"], "zonal": z["s0_zonal"],
+             "meridional": z["s0_merid"]}
+    first.update({k: np.zeros((6, n + 2, n + 2)) for k in capi.HISTORY_CELL_FIELDS})
+    capi.history_write(path, 1, first, dof, n, lib=hip_lib)
+    fields = []
+    with capi.Solver(dof, n, rec["dt"], m, st, scheme=rec["scheme"], lib=hip_lib) as s:
+        for slot in (2, 3):
+            s.step(rec["history_every"])
+            s.history_write(path, slot)
+            fields.append(s.history_fields())
+    V = _read(path)["vars"]
+    assert np.array_equal(V["time"]["data"], [1, 2, 3])
+    for var, key in RECORD.items():
+        assert np.array_equal(V[var]["data"][0], first[key]), var
+        for slot, fl in zip((2, 3), fields):
+            assert V[var]["data"][slot - 1].tobytes() == fl[key].astype(">f8").tobytes(), var
+
+
+HOST_NAMELIST = """&time_settings
+run_days = 0
+run_hours = 2
+dt = 1200.0
+history_interval = 3600
+integral_scheme = 'RK4'
+/
+&case_select
+case_num = 5
+/
+&domain
+dx = 22.5
+dy = 22.5
+/
+&gpu_settings
+dof = 3
+/
+"""
+
+
+def _check_host_program(exe, wd):
+    """The Fortran host program writes the reference's history file: record 1
+    from the initial state, records 2 and 3 after steps 3 and 6, bit for bit
+    against the reference's own records of the same run."""
+    import subprocess
+    name = "history_mcv3_n4_case5_rk4"
+    rec, z, m, st = _history_case(name)
+    assert (rec["dof"], rec["n"], rec["case_num"], rec["scheme"], rec["dt"], rec["steps"],
+            rec["history_every"]) == (3, 4, 5, "RK4", 1200.0, 6, 3)
+    os.makedirs(wd, exist_ok=True)
+    with open(os.path.join(wd, "namelist.input"), "w") as fh:
+        fh.write(HOST_NAMELIST)
+    p = subprocess.run([exe], cwd=wd, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-3000:] + p.stderr[-3000:]
+    f = _read(os.path.join(wd, "mcv_output.nc"))
+    V = f["vars"]
+    assert np.array_equal(V["time"]["data"], [1, 2, 3])
+    c = hostlib.cells(3, 4)
+    assert V["areaCell"]["data"].tobytes() == c["areaCell"].astype(">f8").tobytes()
+    assert V["phiP_s"]["data"].tobytes() == z["phi_s"].astype(">f8").tobytes()
+    cm = corner_mask(3, 4)
+    first = {"u": z["s0_u"], "v": z["s0_v"], "contraU": z["s0_contraU"],
+             "contraV": z["s0_contraV"], "phi": z["s0_phi"], "phi_t": z["s0_phi"] + z["phi_s"],
+             "zonal": z["s0_zonal"], "meridional": z["s0_merid"]}
+    for var, key in RECORD.items():
+        got = V[var]["data"]
+        if key in first:
+            assert got[0].tobytes() == first[key].astype(">f8").tobytes(), (var, 1)
+        else:
+            assert not got[0].any(), var  # cell averages are not formed before record 1
+        for slot, step in ((2, 3), (3, 6)):
+            want = _expected(z, step)[key]
+            g = got[slot - 1]
+            if key in ("contraU", "contraV") and step == 3:
+                g, want = g[:, ~cm], want[:, ~cm]
+            assert g.astype("<f8").tobytes() == want.tobytes(), (var, slot)
+
+
+def test_emulated_host_program_history_file(emu_lib, tmp_path):
+    from mcv_sw_amd import build
+    out = os.path.join(os.path.dirname(HERE), "tests", "emu", "_build", "mcv_sw_gpu_emu")
+    exe = build.build_driver(hip_lib=emu_lib, out=out)
+    _check_host_program(exe, str(tmp_path / "hist"))
+
+
+@pytest.mark.gpu
+def test_host_program_history_file(hip_lib, tmp_path):
+    from mcv_sw_amd import build
+    print("[history] Fortran host program, MCV3 n=4 case 5", flush=True)
+    assert os.path.exists(build.DRIVER), "build the host program with __graft_entry__.build()"
+    _check_host_program(build.DRIVER, str(tmp_path / "hist"))
+
+
+def test_order_check_norms_closed_form():
+    """orderCheck.m:60-82 on manufactured fields: a uniform relative error e
+    gives L1 = L2 = LInf = e; only interior cells count; orders of
+    errors ~ dx^k are k."""
+    from mcv_sw_amd import diagnostics
+    rng = np.random.default_rng(3)
+    n = 8
+    area = hostlib.cells(3, n)["areaCell"]
+    ref = 1.0 + rng.random((6, n + 2, n + 2))
+    e = 1e-3
+    model = ref * (1 + e)
+    model[:, 0, :] = 1e9  # halo ring: ignored
+    l1, l2, linf = diagnostics.norms(model, ref, area)
+    assert abs(l1 - e) < 1e-12 and abs(l2 - e) < 1e-12 and abs(linf - e) < 1e-12
+    assert diagnostics.norms(ref, ref, area) == (0.0, 0.0, 0.0)
+    # one cell off by d: L1 = d A / sum(|r| A), LInf = d A / max(|r| A)
+    m2 = ref.copy()
+    m2[2, 3, 4] += 0.5
+    a = area[2, 3, 4]
+    l1, l2, linf = diagnostics.norms(m2, ref, area)
+    ri, ai = ref[:, 1:-1, 1:-1], area[:, 1:-1, 1:-1]
+    assert abs(l1 - 0.5 * a / np.sum(ri * ai)) < 1e-15
+    assert abs(l2 - np.sqrt(0.25 * a / np.sum(ri * ri * ai))) < 1e-15
+    assert abs(linf - 0.5 * a / np.max(ri * ai)) < 1e-15
+    assert np.allclose(diagnostics.orders([8.0, 1.0, 0.125], [4.0, 2.0, 1.0]), [3.0, 3.0])
+
+
+@pytest.mark.parametrize("dof,sizes,order", [(3, (6, 12, 24), 2.7), (4, (6, 12), 3.7)])
+def test_case2_converges_at_design_order(dof, sizes, order, emu_lib):
+    """Williamson case 2 is steady: after one day the cell-averaged phi
+    differs from the initial one by the scheme's error, which falls at the
+    MCV order (3 for MCV3, 4 for MCV4) as the mesh is refined; device cell
+    averages and the orderCheck norms measure it."""
+    from mcv_sw_amd import diagnostics
+    errs, dxs = [], []
+    for n in sizes:
+        m = hostlib.mesh(dof, n)
+        st = hostlib.williamson(m, 2)
+        dt = 1200.0 / (n // 6)
+        with capi.Solver(dof, n, dt, m, st, lib=emu_lib) as s:
+            r0 = s.cell_averages()[2]
+            s.step(int(round(86400 / dt)))
+            r1 = s.cell_averages()[2]
+        errs.append(diagnostics.norms(r1, r0, hostlib.cells(dof, n)["areaCell"])[1])
+        dxs.append(90.0 / n)
+    assert diagnostics.orders(errs, dxs)[-1] > order, errs
+
+
+def test_accuracy_tool_reports_ordercheck_norms(emu_lib, capsys):
+    """The FP32C-vs-FP64 report carries the orderCheck norms of the device
+    cell averages; FP32C stays within FP32 reach of FP64 on a short run."""
+    from mcv_sw_amd import accuracy
+    accuracy.main(["--dof", "3", "--n", "6", "--dt", "600", "--days", "0.05,0.1",
+                   "--lmax", "4", "--rms", "20", "--u0", "20", "--h0", "5960",
+                   "--lib", emu_lib])
+    lines = [json.loads(x) for x in capsys.readouterr().out.splitlines() if x.startswith("{")]
+    assert [r["steps"] for r in lines] == [7, 14]
+    for r in lines:
+        oc = r["ordercheck_fp32c"]
+        assert set(oc) == {"h", "u", "v"}
+        assert 0 < oc["h"]["L2"] < 1e-6 and oc["h"]["L1"] <= oc["h"]["LInf"] * 10
