@@ -195,6 +195,9 @@ def run_single(capi, m, st, precision, steps, warmup, history=None):
     return rec, out
 
 
+GRAPHS = True  # batch stage sequences replayed as hipGraphs (--no-graphs: direct launches)
+
+
 def run_ensemble(capi, hostlib, stages, warmup_stages):
     """The uniform ensemble in one batch; cell-steps/s of all members."""
     m = hostlib.mesh(E_DOF, E_N)
@@ -202,6 +205,7 @@ def run_ensemble(capi, hostlib, stages, warmup_stages):
     nst = 3  # RK3 stages per step
     total_steps = (warmup_stages + stages + 2 * stages) // nst + 2
     with capi.Batch(E_MEMBERS) as b:
+        b.set_graphs(GRAPHS)
         t0 = time.time()
         for k in range(E_MEMBERS):
             st = hostlib.jet(m, u0=E_U0, h0=E_H0, lmax=E_LMAX, seed=k + 1, rms_h=E_RMS)
@@ -261,6 +265,7 @@ def run_uneven(capi, hostlib, count=128, stages=24, seed=0):
     steps = stages // 4
     cells = 0
     with capi.Batch(count) as b:
+        b.set_graphs(GRAPHS)
         for sp, (m, st) in zip(specs, states):
             dt = serve.cfl_dt(sp["dof"], sp["n"], sp["u0"], 5960.0)
             b.add(sp["dof"], sp["n"], dt, m, st, 1000, scheme="RK4", precision=capi.FP32C)
@@ -343,7 +348,12 @@ def main():
                     help="seconds of Poisson arrivals for the serving run (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="batches launch every kernel directly instead of replaying hipGraphs "
+                         "(for kernel-trace profiling)")
     args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = not args.no_graphs
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the MCV step here is a "
               "single-GPU build.", file=sys.stderr)

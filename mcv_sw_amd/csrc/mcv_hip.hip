@@ -36,7 +36,8 @@ size_t footprint_bytes(const mcv_config *c) {
   const size_t H = c->dof - 1, npv = (size_t)c->n * H + 1, nh = npv + 2 * H;
   const size_t np = 6 * nh * nh, nband = band_values(c->dof, (int)npv), pad = 256;
   const size_t ncell = 6 * (size_t)c->n * c->n;
-  const size_t diag_blocks = std::min<size_t>((ncell + kDiagThreads - 1) / kDiagThreads, 1024);
+  const size_t diag_blocks =
+      std::min<size_t>((ncell + kDiagThreads - 1) / kDiagThreads, kDiagMaxBlocks);
   auto d = [&](size_t count) { return count * sizeof(double) + pad; };
   auto f = [&](size_t count) { return count * sizeof(float) + pad; };
   // static mesh data (panel-invariant metric terms counted on every panel)
@@ -333,7 +334,7 @@ int run_diag(mcv_handle *h, double *mass, double *energy) {
     hipLaunchKernelGGL(diag_partial_kernel<4>, grid, block, 0, h->stream, h->mesh, y[0], y[1],
                        y[2], cu0, cv0, h->partials);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(1), 0, h->stream, h->partials,
+  hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, h->stream, h->partials,
                      h->diag_blocks, h->diag_out);
   HIP_TRY(hipGetLastError());
   double r[2];
@@ -605,7 +606,8 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   if (h->fp32c && (rc = setup32(h))) return rc;
   // integrals
   const long long ncell = 6LL * cfg->n * cfg->n;
-  h->diag_blocks = (int)std::min<long long>((ncell + kDiagThreads - 1) / kDiagThreads, 1024);
+  h->diag_blocks =
+      (int)std::min<long long>((ncell + kDiagThreads - 1) / kDiagThreads, kDiagMaxBlocks);
   if ((rc = dev_alloc(h, &h->partials, 4 * (size_t)h->diag_blocks))) return rc;
   if ((rc = dev_alloc(h, &h->diag_out, 2))) return rc;
   for (int f = 0; f < 6; ++f)

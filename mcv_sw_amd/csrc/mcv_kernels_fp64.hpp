@@ -5,11 +5,13 @@
 //                    of (phi, u, v) in LDS with the derived E, phiG, contra
 //                    winds; Riemann derivatives at cell ends; interior
 //                    derivatives; tendencies; RK update.  Writes the raw
-//                    updated interior and a raw copy of the four edge lines.
-//   boundary_kernel  per (panel, side, along-edge point): ghost interpolation
-//                    into the halo and edge/vertex averaging (the reference's
-//                    correct_bdy_ghost), reading the raw copy so that no
-//                    thread reads a point another thread rewrites.
+//                    updated interior and the band (a raw copy of the H+1
+//                    lines nearest each panel side).
+//   boundary_kernel  one item per ghost point, cube-edge point or cube
+//                    vertex: ghost interpolation into the halo and edge/vertex
+//                    averaging (the reference's correct_bdy_ghost), reading
+//                    the band so that no thread reads a point another thread
+//                    rewrites.
 // phiG is never stored: every state keeps phiG = phi*sqrtG exactly
 // (time_scheme_mod.F90:85), so it is recomputed from phi where needed.
 #pragma once
@@ -327,12 +329,18 @@ __global__ __launch_bounds__(kDiagThreads) void diag_partial_kernel(MeshDev m, c
   }
 }
 
+// The block partials (at most kDiagMaxBlocks) are staged in LDS by the whole
+// workgroup, then summed by one thread in block order.
+constexpr int kDiagMaxBlocks = 1024;
 static __global__ void diag_final_kernel(const double *partials, int nblocks, double *out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  __shared__ double sp[4 * kDiagMaxBlocks];
+  for (int k = threadIdx.x; k < 4 * nblocks; k += blockDim.x) sp[k] = partials[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
     DD ms{0.0, 0.0}, es{0.0, 0.0};
     for (int b = 0; b < nblocks; ++b) {
-      ms = dd_add(ms, DD{partials[4 * b], partials[4 * b + 1]});
-      es = dd_add(es, DD{partials[4 * b + 2], partials[4 * b + 3]});
+      ms = dd_add(ms, DD{sp[4 * b], sp[4 * b + 1]});
+      es = dd_add(es, DD{sp[4 * b + 2], sp[4 * b + 3]});
     }
     out[0] = ms.hi + ms.lo;
     out[1] = es.hi + es.lo;
