@@ -198,6 +198,23 @@ def run_single(capi, m, st, precision, steps, warmup, history=None):
 GRAPHS = True  # batch stage sequences replayed as hipGraphs (--no-graphs: direct launches)
 
 
+def fp64_check(capi, b, slots, specs, steps_of):
+    """FP32C members of a batch against the same solve run alone in FP64 for
+    the same number of steps: h/u/v distance and mass drift.  `specs[slot]`
+    is (dof, n, dt, mesh, state, scheme, mass0)."""
+    out = []
+    for slot in slots:
+        dof, n, dt, m, st, scheme, mass0 = specs[slot]
+        steps = steps_of(slot)
+        got = b.sync(slot, fields=("phi", "u", "v"))
+        with capi.Solver(dof, n, dt, m, st, scheme=scheme) as s:
+            s.step(steps)
+            ref = s.sync(fields=("phi", "u", "v"))
+        e = distance(ref, got, dof, n, mass0)
+        out.append({"slot": slot, "dof": dof, "n": n, "steps": int(steps), **e})
+    return out
+
+
 def run_ensemble(capi, hostlib, stages, warmup_stages):
     """The uniform ensemble in one batch; cell-steps/s of all members."""
     m = hostlib.mesh(E_DOF, E_N)
@@ -207,9 +224,15 @@ def run_ensemble(capi, hostlib, stages, warmup_stages):
     with capi.Batch(E_MEMBERS) as b:
         b.set_graphs(GRAPHS)
         t0 = time.time()
+        checked = (0, E_MEMBERS - 1)
+        specs = {}
         for k in range(E_MEMBERS):
             st = hostlib.jet(m, u0=E_U0, h0=E_H0, lmax=E_LMAX, seed=k + 1, rms_h=E_RMS)
-            b.add(E_DOF, E_N, E_DT, m, st, total_steps, scheme=E_SCHEME, precision=capi.FP32C)
+            slot = b.add(E_DOF, E_N, E_DT, m, st, total_steps, scheme=E_SCHEME,
+                         precision=capi.FP32C)
+            if k in checked:
+                specs[slot] = (E_DOF, E_N, E_DT, m, st, E_SCHEME,
+                               b.sync(slot, fields=())["mass"])
         log(f"ensemble: {E_MEMBERS} members admitted in {time.time() - t0:.1f} s")
         tile, tile_ms = b.autotune(reps=3)
         b.run(warmup_stages)
@@ -220,6 +243,7 @@ def run_ensemble(capi, hostlib, stages, warmup_stages):
         wall = time.perf_counter() - t
         stage_ms, bnd_ms = b.run_timed(stages)
         left, done = b.poll()
+        errs = fp64_check(capi, b, sorted(specs), specs, lambda sl: done[sl])
     steps = stages / nst
     bpps = fp32c_bytes_per_point_step(E_SCHEME)
     pts = E_MEMBERS * 6 * (E_N * (E_DOF - 1) + 1) ** 2
@@ -233,7 +257,8 @@ def run_ensemble(capi, hostlib, stages, warmup_stages):
             "stage_kernel_GBs": achieved, "stage_kernel_frac": achieved / PEAK_HBM_GBS,
             "algorithmic_bytes_per_point_step": bpps,
             "tile_variant": tile, "tile_variant_ms_per_stage": tile_ms,
-            "members_done_steps": int(done[:E_MEMBERS].min())}
+            "members_done_steps": int(done[:E_MEMBERS].min()),
+            "fp32c_error_vs_fp64_alone": errs}
 
 
 def run_uneven(capi, hostlib, count=128, stages=24, seed=0):
@@ -264,12 +289,21 @@ def run_uneven(capi, hostlib, count=128, stages=24, seed=0):
     log(f"uneven batch: {count} initial states in {time.time() - t0:.1f} s")
     steps = stages // 4
     cells = 0
+    # the smallest solve of each order is checked against FP64 alone
+    chk = {}
+    for k, sp in enumerate(specs):
+        if sp["dof"] not in chk or sp["n"] < specs[chk[sp["dof"]]]["n"]:
+            chk[sp["dof"]] = k
     with capi.Batch(count) as b:
         b.set_graphs(GRAPHS)
-        for sp, (m, st) in zip(specs, states):
+        cspecs, slot_of = {}, {}
+        for k, (sp, (m, st)) in enumerate(zip(specs, states)):
             dt = serve.cfl_dt(sp["dof"], sp["n"], sp["u0"], 5960.0)
-            b.add(sp["dof"], sp["n"], dt, m, st, 1000, scheme="RK4", precision=capi.FP32C)
+            slot = b.add(sp["dof"], sp["n"], dt, m, st, 1000, scheme="RK4", precision=capi.FP32C)
             cells += 6 * sp["n"] ** 2
+            if k in chk.values():
+                cspecs[slot] = (sp["dof"], sp["n"], dt, m, st, "RK4",
+                                b.sync(slot, fields=())["mass"])
         del states
         tile, tile_ms = b.autotune(reps=3)
         b.run(8)
@@ -279,6 +313,8 @@ def run_uneven(capi, hostlib, count=128, stages=24, seed=0):
         b.poll()
         wall = time.perf_counter() - t
         stage_ms, bnd_ms = b.run_timed(stages)
+        _, done = b.poll()
+        errs = fp64_check(capi, b, sorted(cspecs), cspecs, lambda sl: done[sl])
     mix = {}
     for sp in specs:
         k = f"mcv{sp['dof']}_n{sp['n']}"
@@ -290,7 +326,8 @@ def run_uneven(capi, hostlib, count=128, stages=24, seed=0):
             "ms_per_step": wall * 1e3 / steps, "stages_timed": stages,
             "stage_kernel_ms_per_step": stage_ms / steps,
             "boundary_kernel_ms_per_step": bnd_ms / steps,
-            "tile_variant": tile, "tile_variant_ms_per_stage": tile_ms, "mix": mix}
+            "tile_variant": tile, "tile_variant_ms_per_stage": tile_ms, "mix": mix,
+            "fp32c_error_vs_fp64_alone": errs}
 
 
 def run_serve(capi, seconds, rate=20.0, seed=0, check=6):
