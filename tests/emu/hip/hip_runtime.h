@@ -41,6 +41,16 @@ inline unsigned atomicAdd(unsigned *p, unsigned v) {
   *p = o + v;
   return o;
 }
+inline int atomicAdd(int *p, int v) {
+  const int o = *p;
+  *p = o + v;
+  return o;
+}
+inline int atomicExch(int *p, int v) {
+  const int o = *p;
+  *p = v;
+  return o;
+}
 using std::max;
 using std::min;
 
