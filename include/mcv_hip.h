@@ -46,6 +46,10 @@ enum {
   MCV_FP32C = 1 /* FP32 bulk with FP64 metrics, ghost weights and integrals */
 };
 
+/* Earth radius, gravity and the rotation rate enter the step only through the
+ * host's arrays (metric terms and A/IA carry the radius, f carries Omega, the
+ * state is the geopotential phi = g h), so the configuration needs none of
+ * them; the halo width is H = dof - 1. */
 typedef struct mcv_config {
   int32_t abi_version; /* MCV_ABI_VERSION */
   int32_t dof;         /* 3 (MCV3) or 4 (MCV4): constants_mod.F90:4-10 */

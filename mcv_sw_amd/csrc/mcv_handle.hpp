@@ -15,6 +15,9 @@ namespace mcv {
 
 // Records the message returned by mcv_hip_last_error() and returns code.
 int fail(int code, const char *fmt, ...);
+// mcv_history_write that also checks the file's per-record sizes (mcv_netcdf.cpp)
+int history_write_checked(const char *path, int32_t time_slot, const mcv_history_fields *rec,
+                          long long npoint, long long ncell);
 
 #define HIP_TRY(expr)                                                                   \
   do {                                                                                  \

@@ -796,7 +796,7 @@ int mcv_hip_history_write(mcv_handle *h, const char *path, int32_t time_slot) {
     off += k < 8 ? (size_t)h->np : nc;
   }
   if ((rc = download_history(h, &f))) return rc;
-  return mcv_history_write(path, time_slot, &f);
+  return history_write_checked(path, time_slot, &f, h->np, (long long)nc);
 }
 
 int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG, double *t_u,

@@ -29,6 +29,8 @@
 
 namespace mcv {
 int fail(int code, const char *fmt, ...);
+int history_write_checked(const char *path, int32_t time_slot, const mcv_history_fields *rec,
+                          long long npoint, long long ncell);
 }
 
 namespace {
@@ -339,6 +341,17 @@ int mcv_history_init(const char *path, const mcv_history_info *info, const doubl
 }
 
 int mcv_history_write(const char *path, int32_t time_slot, const mcv_history_fields *rec) {
+  return mcv::history_write_checked(path, time_slot, rec, -1, -1);
+}
+
+}  // extern "C"
+
+namespace mcv {
+
+// mcv_history_write; with npoint, ncell > 0 the file's record variables must
+// hold exactly that many values per record (the caller's array sizes).
+int history_write_checked(const char *path, int32_t time_slot, const mcv_history_fields *rec,
+                          long long npoint, long long ncell) {
   if (!path || !rec) return fail(MCV_EINVAL, "mcv_history_write: NULL argument");
   if (time_slot < 1) return fail(MCV_EINVAL, "time_slot must be >= 1, got %d", time_slot);
   const double *src[11] = {rec->u, rec->v, rec->contraU, rec->contraV, rec->phi, rec->phi_t,
@@ -376,6 +389,10 @@ int mcv_history_write(const char *path, int32_t time_slot, const mcv_history_fie
       rc = fail(MCV_EINVAL, "%s has no record variable %s", path, kRecordNames[k]);
     else if (rv[k]->type != (k == 0 ? (uint32_t)kNcInt : (uint32_t)kNcDouble))
       rc = fail(MCV_EINVAL, "%s: variable %s has an unexpected type", path, kRecordNames[k]);
+    else if (k > 0 && npoint > 0 &&
+             (long long)(rv[k]->vsize / 8) != (k <= 8 ? npoint : ncell))
+      rc = fail(MCV_EINVAL, "%s: variable %s holds %lld values per record, this solve %lld",
+                path, kRecordNames[k], (long long)(rv[k]->vsize / 8), k <= 8 ? npoint : ncell);
   }
   if (!rc) {
     const uint64_t rec0 = (uint64_t)(time_slot - 1) * h.recsize;
@@ -395,4 +412,5 @@ int mcv_history_write(const char *path, int32_t time_slot, const mcv_history_fie
   return rc;
 }
 
-}  // extern "C"
+}  // namespace mcv
+

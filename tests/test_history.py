@@ -389,3 +389,26 @@ def test_accuracy_tool_reports_ordercheck_norms(emu_lib, capsys):
         oc = r["ordercheck_fp32c"]
         assert set(oc) == {"h", "u", "v"}
         assert 0 < oc["h"]["L2"] < 1e-6 and oc["h"]["L1"] <= oc["h"]["LInf"] * 10
+
+
+def _check_rejects_other_mesh(lib, tmp_path):
+    rec, z, m, st = _history_case("history_mcv3_n4_case5_rk4")
+    path = str(tmp_path / "other.nc")
+    m6 = hostlib.mesh(3, 6)
+    capi.history_init(path, 3, 6, rec["dt"], hostlib.cells(3, 6), m6["lonP"], m6["latP"],
+                      np.zeros_like(m6["sqrtG"]), lib=lib)
+    with capi.Solver(3, 4, rec["dt"], m, st, scheme=rec["scheme"], lib=lib) as s:
+        s.step(1)
+        with pytest.raises(capi.MCVError) as ei:
+            s.history_write(path, 2)
+    assert ei.value.code == capi.MCV_EINVAL and "values per record" in str(ei.value)
+
+
+def test_emulated_history_write_rejects_other_mesh(emu_lib, tmp_path):
+    """A record of one mesh is not written into another mesh's file."""
+    _check_rejects_other_mesh(emu_lib, tmp_path)
+
+
+@pytest.mark.gpu
+def test_history_write_rejects_other_mesh(hip_lib, tmp_path):
+    _check_rejects_other_mesh(hip_lib, tmp_path)
