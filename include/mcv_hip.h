@@ -146,7 +146,9 @@ int mcv_hip_copy_probe(int32_t device, int64_t bytes, int32_t reps, double *ms_p
 int mcv_hip_free(mcv_handle *h);
 
 /* Device bytes a solve of this configuration allocates at most (no
- * counterpart in the reference; the admission test of the batch API). */
+ * counterpart in the reference; the admission test of the batch API), not
+ * counting the history buffers that the first mcv_hip_cell_averages (3 cell
+ * fields) or mcv_hip_history_* call (8 point fields more) adds. */
 int mcv_hip_footprint(const mcv_config *cfg, int64_t *bytes);
 
 /* Device bytes the handle holds. */
