@@ -48,6 +48,11 @@ struct MCVDivisors {
 // metric terms sqrtG and IG are identical on all six panels of the reference
 // mesh; metric_pstride is 0 then (one copy serves every panel) and nh*nh
 // otherwise.  The same holds for phi_s with phis_pstride.
+// Two doubles loaded together (one 16-byte load).
+struct alignas(16) D2 {
+  double a, b;
+};
+
 struct MeshDev {
   int dof, n, npv, nh;
   double dx;                 // element width, radians
@@ -55,6 +60,11 @@ struct MeshDev {
   long long metric_pstride;  // panel stride of sqrtG, rsqrtG, ig11, ig12, ig22
   long long phis_pstride;    // panel stride of phi_s
   const double *sqrtG, *rsqrtG, *ig11, *ig12, *ig22;  // rsqrtG = RN(1/sqrtG)
+  // The same metric terms packed per point for the stage kernel, so that a
+  // point's terms come in one or two 16-byte loads (metric_pstride applies):
+  const D2 *mpack;  // 2 per point: {sqrtG, IG11}, {IG12, IG22}
+  const D2 *spack;  // 1 per point: {sqrtG, 1/sqrtG}
+  int phis_zero;    // phi_s is +0.0 at every point (not read; adding +0.0 is exact)
   const double *f;           // per panel (f differs between panels 5 and 6)
   const double *phi_s;
   const double *p2sp;        // (A*IG) per point, 4 doubles (m11, m21, m12, m22)

@@ -43,6 +43,7 @@ size_t footprint_bytes(const mcv_config *c) {
   // static mesh data (panel-invariant metric terms counted on every panel)
   size_t b = 5 * d(np) + d(np) + d(np) + d(4 * np) + d(H * npv) +
              sizeof(int32_t) * H * npv + pad;
+  b += d(4 * np) + d(2 * np);  // packed metric terms
   b += d(4 * 6 * 4 * (H + 1) * npv) + d(4 * 6 * 4 * H * npv) + d(4 * 6 * 4 * npv);  // band tables
   const bool fp32c = c->precision == MCV_FP32C;
   b += (fp32c ? 3 : 9) * d(np);                       // FP64 levels
@@ -532,6 +533,27 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   m.ig12 = d;
   if ((rc = upload(h, &d, ig22.data(), mcount))) return rc;
   m.ig22 = d;
+  {
+    std::vector<double> mp(4 * mcount), sp(2 * mcount);
+    for (long long g = 0; g < mcount; ++g) {
+      mp[4 * g] = mesh->sqrtG[g];
+      mp[4 * g + 1] = ig11[g];
+      mp[4 * g + 2] = ig12[g];
+      mp[4 * g + 3] = ig22[g];
+      sp[2 * g] = mesh->sqrtG[g];
+      sp[2 * g + 1] = rsg[g];
+    }
+    if ((rc = upload(h, &d, mp.data(), mp.size()))) return rc;
+    m.mpack = reinterpret_cast<const D2 *>(d);
+    if ((rc = upload(h, &d, sp.data(), sp.size()))) return rc;
+    m.spack = reinterpret_cast<const D2 *>(d);
+  }
+  m.phis_zero = 1;
+  for (long long g = 0; g < np && m.phis_zero; ++g) {
+    uint64_t bits;
+    std::memcpy(&bits, &mesh->phi_s[g], sizeof bits);
+    if (bits != 0) m.phis_zero = 0;
+  }
   if ((rc = upload(h, &d, mesh->f, np))) return rc;
   m.f = d;
   if ((rc = upload(h, &d, mesh->phi_s, phis_inv ? per : np))) return rc;

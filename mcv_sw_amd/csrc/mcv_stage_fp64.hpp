@@ -72,6 +72,8 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
   const uint32_t pbase = (uint32_t)(p - 1) * (uint32_t)(nh * nh);
   const uint32_t mbase = (uint32_t)(p - 1) * (uint32_t)m.metric_pstride;
   const uint32_t sbase = (uint32_t)(p - 1) * (uint32_t)m.phis_pstride;
+  const D2 *const mpk = m.mpack + 2 * (size_t)mbase;  // this panel's packed metric terms
+  const D2 *const spk = m.spack + (size_t)mbase;
 
   // Phase 0: stage the cross-shaped tile + halo; derived fields per point.
   // Every thread first issues the loads of all its points, then derives and
@@ -94,11 +96,12 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       rphi[q] = ldg(a.in_phi, g);
       ru[q] = ldg(a.in_u, g);
       rv[q] = ldg(a.in_v, g);
-      rsg[q] = ldg(m.sqrtG, mbase + off);
-      r11[q] = ldg(m.ig11, mbase + off);
-      r12[q] = ldg(m.ig12, mbase + off);
-      r22[q] = ldg(m.ig22, mbase + off);
-      rps[q] = ldg(m.phi_s, sbase + off);
+      const D2 ga = ldg(mpk, 2 * off), gb = ldg(mpk, 2 * off + 1);
+      rsg[q] = ga.a;
+      r11[q] = ga.b;
+      r12[q] = gb.a;
+      r22[q] = gb.b;
+      rps[q] = m.phis_zero ? 0.0 : ldg(m.phi_s, sbase + off);
       if (a.cu0) {
         rcu[q] = ldg(a.cu0, g);
         rcv[q] = ldg(a.cv0, g);
@@ -220,8 +223,9 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       if (!ok) continue;
       const uint32_t off = (uint32_t)((j0 + oy + H - 1) * nh + (i0 + ox + H - 1));
       const uint32_t g = pbase + off;
-      rsg[q] = ldg(m.sqrtG, mbase + off);
-      rrs[q] = ldg(m.rsqrtG, mbase + off);
+      const D2 sr = ldg(spk, off);
+      rsg[q] = sr.a;
+      rrs[q] = sr.b;
       rf[q] = ldg(m.f, g);
       if (need_base) {
         rbp[q] = ldg(a.b_phi, g);
