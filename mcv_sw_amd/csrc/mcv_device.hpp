@@ -48,9 +48,12 @@ struct MCVDivisors {
 // metric terms sqrtG and IG are identical on all six panels of the reference
 // mesh; metric_pstride is 0 then (one copy serves every panel) and nh*nh
 // otherwise.  The same holds for phi_s with phis_pstride.
-// Two doubles loaded together (one 16-byte load).
+// Two doubles loaded together (one 16-byte load); two floats (one 8-byte load).
 struct alignas(16) D2 {
   double a, b;
+};
+struct alignas(8) F2 {
+  float a, b;
 };
 
 struct MeshDev {
@@ -80,7 +83,8 @@ struct MeshDev {
 struct StageArgs {
   const double *in_phi, *in_u, *in_v;  // stage state Y_s (halo filled)
   const double *b_phi, *b_u, *b_v;     // RK base state Y_n
-  double *acc_phiG, *acc_u, *acc_v;    // RK4 tendency accumulator
+  D2 *acc_pu;                          // RK4 tendency accumulator: {phiG, u} per point
+  double *acc_v;                       //   and v
   double *o_phi, *o_u, *o_v;           // stage output (interior, raw)
   double *band_raw;                    // raw copy of the lines nearest each side, see band_slot()
   const double *cu0, *cv0;             // host contravariant winds (first stage only)

@@ -29,9 +29,10 @@ struct MeshF32 {
   double dxf;                   // dx
   long long metric_pstride;
   const double *sg, *rsg, *ig11, *ig12, *ig22;  // FP64 metric (same arrays as the FP64 path)
+  const D2 *mpack, *spack;      // the same terms packed per point (MeshDev::mpack, spack)
   const float *f;               // RN32(Coriolis parameter)
   const float *pgref;           // RN32(phi_ref * sqrtG)
-  const float *dEx, *dEy;       // RN32 of the FP64 MCV derivatives of phi_ref + phi_s
+  const F2 *dE;                 // RN32 of the FP64 MCV derivatives (x, y) of phi_ref + phi_s
 };
 
 struct StageArgs32 {
@@ -111,6 +112,8 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   // 32-bit point indices (every array here is below 2^32 bytes)
   const uint32_t pbase = (uint32_t)(p - 1) * (uint32_t)(nh * nh);
   const uint32_t mbase = (uint32_t)(p - 1) * (uint32_t)m.metric_pstride;
+  const D2 *const mpk = m.mpack + 2 * (size_t)mbase;  // this panel's packed metric terms
+  const D2 *const spk = m.spack + (size_t)mbase;
 
   // loads of all a thread's points first, then the derived values
   constexpr int kP0 = (TX * TY + kStageThreads - 1) / kStageThreads;
@@ -131,11 +134,12 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       ru[q] = ldg(a.in_u, g);
       rv[q] = ldg(a.in_v, g);
       rpg[q] = ldg(m.pgref, g);
-      r11[q] = ldg(m.ig11, mbase + off);
-      r12[q] = ldg(m.ig12, mbase + off);
-      r22[q] = ldg(m.ig22, mbase + off);
-      rsg[q] = ldg(m.sg, mbase + off);
-      rrs[q] = ldg(m.rsg, mbase + off);
+      const D2 ga = ldg(mpk, 2 * off), gb = ldg(mpk, 2 * off + 1), sr = ldg(spk, off);
+      r11[q] = ga.b;
+      r12[q] = gb.a;
+      r22[q] = gb.b;
+      rsg[q] = sr.a;
+      rrs[q] = sr.b;
     }
 #pragma unroll
     for (int q = 0; q < kP0; ++q) {
@@ -255,11 +259,13 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       if (!ok) continue;
       const uint32_t off = (uint32_t)((j0 + oy + H - 1) * nh + (i0 + ox + H - 1));
       const uint32_t g = pbase + off;
-      rsg[q] = ldg(m.sg, mbase + off);
-      rrs[q] = ldg(m.rsg, mbase + off);
+      const D2 sr = ldg(spk, off);
+      rsg[q] = sr.a;
+      rrs[q] = sr.b;
       rf[q] = ldg(m.f, g);
-      rdx_[q] = ldg(m.dEx, g);
-      rdy_[q] = ldg(m.dEy, g);
+      const F2 de = ldg(m.dE, g);
+      rdx_[q] = de.a;
+      rdy_[q] = de.b;
       if (kind == kTend) continue;
       if (last) {
         rcp[q] = ldg(a.c_phi, g);
@@ -584,8 +590,7 @@ static __global__ void band_copy_kernel(MeshDev m, const double *phi, const doub
 // dissipation term left out: the part of dE/dx, dE/dy that does not depend on
 // the evolving state.  One thread per owned point; runs once per solve.
 template <int DOF>
-__global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, float *dEx,
-                                    float *dEy) {
+__global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, F2 *dE) {
   constexpr int H = DOF - 1;
   const int npv = m.npv, nh = m.nh;
   const long long per = (long long)nh * nh;
@@ -616,8 +621,7 @@ __global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, float *dEx
       return cell_interior<DOF>(f, dL, central(bL + H), pos, m.dx, m.dv);
     };
     const long long g = pidx(nh, H, p, i, j);
-    dEx[g] = (float)deriv(true);
-    dEy[g] = (float)deriv(false);
+    dE[g] = F2{(float)deriv(true), (float)deriv(false)};
   }
 }
 

@@ -74,7 +74,7 @@ struct mcv_handle {
   std::vector<void *> allocs;  // every device allocation, freed together
   size_t device_bytes = 0;     // their sizes summed
   double *lv[3][3] = {};       // level (Yn, S1, S2) x field (phi, u, v)
-  double *acc[3] = {};
+  double *acc[3] = {};         // RK4 accumulator: acc[0] {phiG, u} pairs, acc[2] v
   double *band_raw = nullptr;  // raw band lines written by the stage kernel
   double *cu0 = nullptr, *cv0 = nullptr;  // host contra winds until the first stage
   bool first_pending = true;

@@ -58,7 +58,7 @@ StageArgs stage_args64(const mcv_handle *h, int in, int out, double c1) {
   StageArgs a{};
   a.in_phi = h->lv[in][0]; a.in_u = h->lv[in][1]; a.in_v = h->lv[in][2];
   a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
-  a.acc_phiG = h->acc[0]; a.acc_u = h->acc[1]; a.acc_v = h->acc[2];
+  a.acc_pu = reinterpret_cast<D2 *>(h->acc[0]); a.acc_v = h->acc[2];
   a.o_phi = h->lv[out][0]; a.o_u = h->lv[out][1]; a.o_v = h->lv[out][2];
   a.band_raw = h->band_raw;
   a.c1 = c1;
@@ -382,8 +382,9 @@ int setup32(mcv_handle *h) {
   HIP_TRY(hipGetLastError());
   if ((rc = boundary(h, 0))) return rc;
   h->first_pending = false;  // the FP32C stages form the contravariant winds themselves
-  float *dEx, *dEy, *f32;
-  if ((rc = dev_alloc_t(h, &dEx, np)) || (rc = dev_alloc_t(h, &dEy, np)) ||
+  F2 *dE;
+  float *f32;
+  if ((rc = dev_alloc_t(h, &dE, np)) ||
       (rc = dev_alloc_t(h, &f32, np)))
     return rc;
   hipLaunchKernelGGL(round_to_f32_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->mesh.f,
@@ -391,10 +392,10 @@ int setup32(mcv_handle *h) {
   HIP_TRY(hipGetLastError());
   if (h->cfg.dof == 3)
     hipLaunchKernelGGL(ref_gradient_kernel<3>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
-                       h->lv[0][0], dEx, dEy);
+                       h->lv[0][0], dE);
   else
     hipLaunchKernelGGL(ref_gradient_kernel<4>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
-                       h->lv[0][0], dEx, dEy);
+                       h->lv[0][0], dE);
   HIP_TRY(hipGetLastError());
   for (int l = 0; l < 3; ++l)
     for (int f = 0; f < 3; ++f)
@@ -434,10 +435,11 @@ int setup32(mcv_handle *h) {
   m.ig11 = d.ig11;
   m.ig12 = d.ig12;
   m.ig22 = d.ig22;
+  m.mpack = d.mpack;
+  m.spack = d.spack;
   m.f = f32;
   m.pgref = pgref;
-  m.dEx = dEx;
-  m.dEy = dEy;
+  m.dE = dE;
   HIP_TRY(hipStreamSynchronize(h->stream));
   return MCV_OK;
 }
@@ -611,9 +613,10 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   for (int l = 0; l < nlev; ++l)
     for (int f = 0; f < 3; ++f)
       if ((rc = dev_alloc(h, &h->lv[l][f], np))) return rc;
-  if (!h->fp32c)
-    for (int f = 0; f < 3; ++f)
-      if ((rc = dev_alloc(h, &h->acc[f], np))) return rc;
+  if (!h->fp32c) {  // accumulator: {phiG, u} pairs and v
+    if ((rc = dev_alloc(h, &h->acc[0], 2 * np))) return rc;
+    if ((rc = dev_alloc(h, &h->acc[2], np))) return rc;
+  }
   if ((rc = dev_alloc(h, &h->band_raw, band_values(cfg->dof, h->npv)))) return rc;
   HIP_TRY(hipMemcpy(h->lv[0][0], st->phi, np * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->lv[0][1], st->u, np * sizeof(double), hipMemcpyHostToDevice));
@@ -890,7 +893,7 @@ int mcv_hip_autotune(mcv_handle *h, int32_t reps, int32_t *tile, double *ms_per_
   StageArgs a{};
   a.in_phi = h->lv[0][0]; a.in_u = h->lv[0][1]; a.in_v = h->lv[0][2];
   a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
-  a.acc_phiG = h->acc[0]; a.acc_u = h->acc[1]; a.acc_v = h->acc[2];
+  a.acc_pu = reinterpret_cast<D2 *>(h->acc[0]); a.acc_v = h->acc[2];
   a.o_phi = h->lv[1][0]; a.o_u = h->lv[1][1]; a.o_v = h->lv[1][2];
   a.band_raw = h->band_raw;
   if (h->first_pending) { a.cu0 = h->cu0; a.cv0 = h->cv0; }

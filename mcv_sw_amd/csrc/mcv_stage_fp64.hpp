@@ -233,8 +233,9 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
         rbv[q] = ldg(a.b_v, g);
       }
       if (need_acc) {
-        rap[q] = ldg(a.acc_phiG, g);
-        rau[q] = ldg(a.acc_u, g);
+        const D2 pu = ldg(a.acc_pu, g);
+        rap[q] = pu.a;
+        rau[q] = pu.b;
         rav[q] = ldg(a.acc_v, g);
       }
     }
@@ -308,11 +309,11 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
         double pg, u, v;
         if (kind == kRK4S1) {
           pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-          stg(a.acc_phiG, g, kpg); stg(a.acc_u, g, ku); stg(a.acc_v, g, kv);
+          stg(a.acc_pu, g, D2{kpg, ku});
+          stg(a.acc_v, g, kv);
         } else if (kind == kRK4S2 || kind == kRK4S3) {
           pg = pgn + a.c1 * kpg; u = un + a.c1 * ku; v = vn + a.c1 * kv;
-          stg(a.acc_phiG, g, rap[q] + 2.0 * kpg);
-          stg(a.acc_u, g, rau[q] + 2.0 * ku);
+          stg(a.acc_pu, g, D2{rap[q] + 2.0 * kpg, rau[q] + 2.0 * ku});
           stg(a.acc_v, g, rav[q] + 2.0 * kv);
         } else if (kind == kRK4S4) {
           constexpr Recip k6{6.0, 1.0 / 6.0};
