@@ -80,19 +80,25 @@ struct MeshDev {
 };
 
 // One RK stage of one instance: where the stage reads and writes.
+// A time level is stored as {phi, u} pairs and v, so that a point's state
+// comes in two loads (pu[g] = {phi, u}, v[g]), reference layout otherwise.
 struct StageArgs {
-  const double *in_phi, *in_u, *in_v;  // stage state Y_s (halo filled)
-  const double *b_phi, *b_u, *b_v;     // RK base state Y_n
+  const D2 *in_pu;                     // stage state Y_s (halo filled)
+  const double *in_v;
+  const D2 *b_pu;                      // RK base state Y_n
+  const double *b_v;
   D2 *acc_pu;                          // RK4 tendency accumulator: {phiG, u} per point
   double *acc_v;                       //   and v
-  double *o_phi, *o_u, *o_v;           // stage output (interior, raw)
+  D2 *o_pu;                            // stage output (interior, raw); the tendency seam
+  double *o_v;                         //   writes {phiG, u} and v tendencies on (nPV, nPV, 6)
   double *band_raw;                    // raw copy of the lines nearest each side, see band_slot()
   const double *cu0, *cv0;             // host contravariant winds (first stage only)
   double c1;                           // stage increment coefficient (x dt)
 };
 
 struct BoundaryArgs {
-  double *phi, *u, *v;                 // level to finish (halo + edges)
+  D2 *pu;                              // level to finish (halo + edges)
+  double *v;
   const double *band_raw;
 };
 

@@ -570,8 +570,8 @@ static __global__ void pgref_kernel(MeshDev m, const double *phi_ref, float *pgr
 }
 
 // FP64 band copy of a finished level (to run the boundary pass on it).
-static __global__ void band_copy_kernel(MeshDev m, const double *phi, const double *u,
-                                        const double *v, double *band) {
+static __global__ void band_copy_kernel(MeshDev m, const D2 *pu, const double *v,
+                                        double *band) {
   const int npv = m.npv, H = m.dof - 1;
   const int n = 6 * 4 * (H + 1) * npv;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
@@ -580,8 +580,9 @@ static __global__ void band_copy_kernel(MeshDev m, const double *phi, const doub
     int i, j;
     band_ij(npv, side, d, al, i, j);
     const long long g = pidx(m.nh, H, p, i, j);
-    band[band_slot(npv, H, p, side, d, 0, al)] = phi[g];
-    band[band_slot(npv, H, p, side, d, 1, al)] = u[g];
+    const D2 x = pu[g];
+    band[band_slot(npv, H, p, side, d, 0, al)] = x.a;
+    band[band_slot(npv, H, p, side, d, 1, al)] = x.b;
     band[band_slot(npv, H, p, side, d, 2, al)] = v[g];
   }
 }

@@ -73,7 +73,9 @@ struct mcv_handle {
   mcv::MeshDev mesh{};
   std::vector<void *> allocs;  // every device allocation, freed together
   size_t device_bytes = 0;     // their sizes summed
-  double *lv[3][3] = {};       // level (Yn, S1, S2) x field (phi, u, v)
+  mcv::D2 *lpu[3] = {};        // levels (Yn, S1, S2): {phi, u} pairs
+  double *lvv[3] = {};         //   and v
+  double *phi_ref = nullptr;   // FP32C: the projected initial geopotential (FP64)
   double *acc[3] = {};         // RK4 accumulator: acc[0] {phiG, u} pairs, acc[2] v
   double *band_raw = nullptr;  // raw band lines written by the stage kernel
   double *cu0 = nullptr, *cv0 = nullptr;  // host contra winds until the first stage

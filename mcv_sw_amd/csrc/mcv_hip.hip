@@ -50,23 +50,24 @@ size_t footprint_bytes(const mcv_config *c) {
   if (!fp32c) b += 3 * d(np);                          // RK4 accumulator
   b += d(nband) + 2 * d(np) + d(4 * diag_blocks) + d(2) + 6 * d(np);
   if (fp32c)
-    b += 3 * d(np) + 3 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) + 2 * f(nband) + f(np);
+    b += 3 * d(np) + d(np) + 3 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) + 2 * f(nband) +
+         f(np);  // join, phi_ref, FP32 statics, levels, compensation, accumulator, bands
   return b;
 }
 
 StageArgs stage_args64(const mcv_handle *h, int in, int out, double c1) {
   StageArgs a{};
-  a.in_phi = h->lv[in][0]; a.in_u = h->lv[in][1]; a.in_v = h->lv[in][2];
-  a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
+  a.in_pu = h->lpu[in]; a.in_v = h->lvv[in];
+  a.b_pu = h->lpu[0]; a.b_v = h->lvv[0];
   a.acc_pu = reinterpret_cast<D2 *>(h->acc[0]); a.acc_v = h->acc[2];
-  a.o_phi = h->lv[out][0]; a.o_u = h->lv[out][1]; a.o_v = h->lv[out][2];
+  a.o_pu = h->lpu[out]; a.o_v = h->lvv[out];
   a.band_raw = h->band_raw;
   a.c1 = c1;
   return a;
 }
 
 BoundaryArgs boundary_args64(const mcv_handle *h, int level) {
-  return BoundaryArgs{h->lv[level][0], h->lv[level][1], h->lv[level][2], h->band_raw};
+  return BoundaryArgs{h->lpu[level], h->lvv[level], h->band_raw};
 }
 
 StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1) {
@@ -308,12 +309,21 @@ int one_step(mcv_handle *h) {
 
 // FP64 (phi, u, v) of Y_n: the level itself, or for FP32C the reconstruction
 // phi_ref + hi + lo, hi + lo.
+// The FP64 level is unpacked into pack[0] (phi), pack[2] (u) and pack[3] (v),
+// the slots the read-back kernels write those fields to.
 int materialize(mcv_handle *h, const double **out) {
   if (!h->fp32c) {
-    for (int f = 0; f < 3; ++f) out[f] = h->lv[0][f];
+    hipLaunchKernelGGL(unpack_pairs_kernel, dim3(1024), dim3(256), 0, h->stream, h->np,
+                       h->lpu[0], h->pack[0], h->pack[2]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h->pack[3], h->lvv[0], h->np * sizeof(double),
+                           hipMemcpyDeviceToDevice, h->stream));
+    out[0] = h->pack[0];
+    out[1] = h->pack[2];
+    out[2] = h->pack[3];
     return MCV_OK;
   }
-  hipLaunchKernelGGL(join_state_kernel, dim3(1024), dim3(256), 0, h->stream, h->np, h->lv[0][0],
+  hipLaunchKernelGGL(join_state_kernel, dim3(1024), dim3(256), 0, h->stream, h->np, h->phi_ref,
                      h->lv32[0][0], h->lv32[0][1], h->lv32[0][2], h->comp32[0], h->comp32[1],
                      h->comp32[2], h->join[0], h->join[1], h->join[2]);
   HIP_TRY(hipGetLastError());
@@ -372,15 +382,21 @@ int setup32(mcv_handle *h) {
   const size_t nband = band_values(h->cfg.dof, h->npv);
   const int threads = 6 * 4 * h->cfg.dof * h->npv;
   int rc;
-  for (int f = 0; f < 3; ++f) {
+  for (int f = 0; f < 3; ++f)
     if ((rc = dev_alloc(h, &h->join[f], np))) return rc;
-    HIP_TRY(hipMemcpyAsync(h->join[f], h->lv[0][f], np * sizeof(double),
-                           hipMemcpyDeviceToDevice, h->stream));
-  }
+  hipLaunchKernelGGL(unpack_pairs_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->lpu[0],
+                     h->join[0], h->join[1]);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(h->join[2], h->lvv[0], np * sizeof(double), hipMemcpyDeviceToDevice,
+                         h->stream));
   hipLaunchKernelGGL(band_copy_kernel, dim3((threads + 255) / 256), dim3(256), 0, h->stream,
-                     h->mesh, h->lv[0][0], h->lv[0][1], h->lv[0][2], h->band_raw);
+                     h->mesh, h->lpu[0], h->lvv[0], h->band_raw);
   HIP_TRY(hipGetLastError());
   if ((rc = boundary(h, 0))) return rc;
+  if ((rc = dev_alloc(h, &h->phi_ref, np))) return rc;
+  hipLaunchKernelGGL(unpack_pairs_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->lpu[0],
+                     h->phi_ref, nullptr);
+  HIP_TRY(hipGetLastError());
   h->first_pending = false;  // the FP32C stages form the contravariant winds themselves
   F2 *dE;
   float *f32;
@@ -392,10 +408,10 @@ int setup32(mcv_handle *h) {
   HIP_TRY(hipGetLastError());
   if (h->cfg.dof == 3)
     hipLaunchKernelGGL(ref_gradient_kernel<3>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
-                       h->lv[0][0], dE);
+                       h->phi_ref, dE);
   else
     hipLaunchKernelGGL(ref_gradient_kernel<4>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
-                       h->lv[0][0], dE);
+                       h->phi_ref, dE);
   HIP_TRY(hipGetLastError());
   for (int l = 0; l < 3; ++l)
     for (int f = 0; f < 3; ++f)
@@ -408,11 +424,11 @@ int setup32(mcv_handle *h) {
     return rc;
   float *pgref;
   if ((rc = dev_alloc_t(h, &pgref, np))) return rc;
-  hipLaunchKernelGGL(pgref_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh, h->lv[0][0],
+  hipLaunchKernelGGL(pgref_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh, h->phi_ref,
                      pgref);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(split_state_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->join[0],
-                     h->lv[0][0], h->join[1], h->join[2], h->lv32[0][0], h->lv32[0][1],
+                     h->phi_ref, h->join[1], h->join[2], h->lv32[0][0], h->lv32[0][1],
                      h->lv32[0][2], h->comp32[0], h->comp32[1], h->comp32[2]);
   HIP_TRY(hipGetLastError());
   for (int l = 1; l < 3; ++l)
@@ -610,21 +626,28 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
   // state levels (FP32C keeps only level 0 in FP64: the reference state)
   h->fp32c = cfg->precision == MCV_FP32C;
   const int nlev = h->fp32c ? 1 : 3;
-  for (int l = 0; l < nlev; ++l)
-    for (int f = 0; f < 3; ++f)
-      if ((rc = dev_alloc(h, &h->lv[l][f], np))) return rc;
+  for (int l = 0; l < nlev; ++l) {
+    double *pu;
+    if ((rc = dev_alloc(h, &pu, 2 * np))) return rc;
+    h->lpu[l] = reinterpret_cast<D2 *>(pu);
+    if ((rc = dev_alloc(h, &h->lvv[l], np))) return rc;
+  }
   if (!h->fp32c) {  // accumulator: {phiG, u} pairs and v
     if ((rc = dev_alloc(h, &h->acc[0], 2 * np))) return rc;
     if ((rc = dev_alloc(h, &h->acc[2], np))) return rc;
   }
   if ((rc = dev_alloc(h, &h->band_raw, band_values(cfg->dof, h->npv)))) return rc;
-  HIP_TRY(hipMemcpy(h->lv[0][0], st->phi, np * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->lv[0][1], st->u, np * sizeof(double), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->lv[0][2], st->v, np * sizeof(double), hipMemcpyHostToDevice));
+  {
+    std::vector<D2> pu(np);
+    for (long long g = 0; g < np; ++g) pu[g] = D2{st->phi[g], st->u[g]};
+    HIP_TRY(hipMemcpy(h->lpu[0], pu.data(), np * sizeof(D2), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemcpy(h->lvv[0], st->v, np * sizeof(double), hipMemcpyHostToDevice));
   // halo corners of the scratch levels are never read, but keep them defined
-  for (int l = 1; l < nlev; ++l)
-    for (int f = 0; f < 3; ++f)
-      HIP_TRY(hipMemcpy(h->lv[l][f], h->lv[0][f], np * sizeof(double), hipMemcpyDeviceToDevice));
+  for (int l = 1; l < nlev; ++l) {
+    HIP_TRY(hipMemcpy(h->lpu[l], h->lpu[0], np * sizeof(D2), hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(h->lvv[l], h->lvv[0], np * sizeof(double), hipMemcpyDeviceToDevice));
+  }
   if ((rc = upload(h, &h->cu0, st->contraU, np))) return rc;
   if ((rc = upload(h, &h->cv0, st->contraV, np))) return rc;
   h->first_pending = true;
@@ -834,31 +857,39 @@ int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG, double 
     return fail(MCV_EINVAL, "the tendency seam evaluates the FP64 operator; this handle is FP32C");
   HIP_TRY(hipSetDevice(h->cfg.device));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  const size_t b = h->np * sizeof(double);
-  // scratch: pack buffers hold the input, the acc level receives the tendency
-  HIP_TRY(hipMemcpy(h->pack[0], in->phi, b, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->pack[2], in->u, b, hipMemcpyHostToDevice));
+  const long long np = h->np, nt = 6LL * h->npv * h->npv;
+  const size_t b = np * sizeof(double);
+  // scratch: the input as a level ({phi, u} pairs in a temporary, v and the
+  // contravariant winds in the pack buffers), the tendencies in temporaries
+  std::vector<D2> pu(np);
+  for (long long g = 0; g < np; ++g) pu[g] = D2{in->phi[g], in->u[g]};
+  double *tin, *tpu, *tv;
+  int rc;
+  if ((rc = dev_alloc(h, &tin, 2 * np)) || (rc = dev_alloc(h, &tpu, 2 * nt)) ||
+      (rc = dev_alloc(h, &tv, nt)))
+    return rc;
+  HIP_TRY(hipMemcpy(tin, pu.data(), np * sizeof(D2), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->pack[3], in->v, b, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->pack[4], in->contraU, b, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->pack[5], in->contraV, b, hipMemcpyHostToDevice));
   StageArgs a{};
-  a.in_phi = h->pack[0]; a.in_u = h->pack[2]; a.in_v = h->pack[3];
+  a.in_pu = reinterpret_cast<const D2 *>(tin); a.in_v = h->pack[3];
   a.cu0 = h->pack[4]; a.cv0 = h->pack[5];
-  double *tk[3];
-  int rc;
-  for (int f = 0; f < 3; ++f)
-    if ((rc = dev_alloc(h, &tk[f], (size_t)6 * h->npv * h->npv))) return rc;
-  a.o_phi = tk[0]; a.o_u = tk[1]; a.o_v = tk[2];
+  a.o_pu = reinterpret_cast<D2 *>(tpu); a.o_v = tv;
   if ((rc = stage(h, kTend, a))) return rc;
-  const size_t tb = (size_t)6 * h->npv * h->npv * sizeof(double);
-  HIP_TRY(hipMemcpyAsync(t_phiG, tk[0], tb, hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipMemcpyAsync(t_u, tk[1], tb, hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipMemcpyAsync(t_v, tk[2], tb, hipMemcpyDeviceToHost, h->stream));
+  std::vector<D2> tk(nt);
+  HIP_TRY(hipMemcpyAsync(tk.data(), tpu, nt * sizeof(D2), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(t_v, tv, nt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  for (int f = 0; f < 3; ++f) {
-    (void)hipFree(tk[f]);
+  for (long long g = 0; g < nt; ++g) {
+    t_phiG[g] = tk[g].a;
+    t_u[g] = tk[g].b;
+  }
+  const long long sizes[3] = {nt, 2 * nt, 2 * np};  // freed in reverse order of allocation
+  for (long long c : sizes) {
+    (void)hipFree(h->allocs.back());
     h->allocs.pop_back();
-    h->device_bytes -= (size_t)6 * h->npv * h->npv * sizeof(double) + 256;
+    h->device_bytes -= (size_t)c * sizeof(double) + 256;
   }
   return MCV_OK;
 }
@@ -891,10 +922,10 @@ int mcv_hip_autotune(mcv_handle *h, int32_t reps, int32_t *tile, double *ms_per_
   // accumulator): S1, the accumulator and the raw edge copy are rewritten by
   // the first stage of every step, so timing never changes the trajectory.
   StageArgs a{};
-  a.in_phi = h->lv[0][0]; a.in_u = h->lv[0][1]; a.in_v = h->lv[0][2];
-  a.b_phi = h->lv[0][0]; a.b_u = h->lv[0][1]; a.b_v = h->lv[0][2];
+  a.in_pu = h->lpu[0]; a.in_v = h->lvv[0];
+  a.b_pu = h->lpu[0]; a.b_v = h->lvv[0];
   a.acc_pu = reinterpret_cast<D2 *>(h->acc[0]); a.acc_v = h->acc[2];
-  a.o_phi = h->lv[1][0]; a.o_u = h->lv[1][1]; a.o_v = h->lv[1][2];
+  a.o_pu = h->lpu[1]; a.o_v = h->lvv[1];
   a.band_raw = h->band_raw;
   if (h->first_pending) { a.cu0 = h->cu0; a.cv0 = h->cv0; }
   a.c1 = 0.5 * h->cfg.dt;

@@ -63,10 +63,10 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
     else { hi = s; hj = npv + k; }
     const long long g = pidx(nh, H, p, hi, hj);
     const double zh = ghost_interp<DOF>(qz, cf, dx), mh = ghost_interp<DOF>(qm, cf, dx);
-    a.phi[g] = ghost_interp<DOF>(qp, cf, dx);
+    const double ph = ghost_interp<DOF>(qp, cf, dx);
     double uh, vh;
     apply2(m.sp2p_ghost + 4 * ghost_point(npv, H, p, side, k, s), zh, mh, uh, vh);
-    a.u[g] = uh;
+    a.pu[g] = D2{ph, uh};
     a.v[g] = vh;
   } else if (t < nghost + nedge) {
     // Edge averaging (spatial_operators_mod.F90:298-345), vertices excluded.
@@ -93,8 +93,8 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
       double u1, v1, u2, v2;
       apply2(m.sp2p_edge + 4 * edge_point(npv, ep.pa, ep.sa, s), zu, mu, u1, v1);
       apply2(m.sp2p_edge + 4 * edge_point(npv, ep.pb, ep.sb, sb), zu, mu, u2, v2);
-      a.phi[ga] = ph; a.u[ga] = u1; a.v[ga] = v1;
-      a.phi[gb] = ph; a.u[gb] = u2; a.v[gb] = v2;
+      a.pu[ga] = D2{ph, u1}; a.v[ga] = v1;
+      a.pu[gb] = D2{ph, u2}; a.v[gb] = v2;
     }
   } else {
     // Cube vertices (spatial_operators_mod.F90:347-386), reference summation order.
@@ -121,8 +121,7 @@ __device__ __forceinline__ void boundary_item(const MeshDev &m, const BoundaryAr
     for (int l = 0; l < 3; ++l) {
       double u, v;
       apply2(m.sp2p_edge + 4 * eg[l], zv, mv, u, v);
-      a.phi[gg[l]] = pv;
-      a.u[gg[l]] = u;
+      a.pu[gg[l]] = D2{pv, u};
       a.v[gg[l]] = v;
     }
   }
@@ -172,6 +171,17 @@ static __global__ void sphere_winds_kernel(MeshDev m, const double *u, const dou
     apply2(m.p2sp + 4 * g, u[g], v[g], z, w);
     zon[g] = z;
     mer[g] = w;
+  }
+}
+
+// {a, b} pairs of a time level into plain arrays (read-back paths; b may be
+// null).
+static __global__ void unpack_pairs_kernel(long long n, const D2 *pu, double *a, double *b) {
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < n;
+       g += (long long)gridDim.x * blockDim.x) {
+    const D2 x = pu[g];
+    a[g] = x.a;
+    if (b) b[g] = x.b;
   }
 }
 

@@ -93,8 +93,9 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       if (!live[q]) continue;
       const uint32_t off = (uint32_t)((j0 - 1 + ly) * nh + (i0 - 1 + lx));
       const uint32_t g = pbase + off;
-      rphi[q] = ldg(a.in_phi, g);
-      ru[q] = ldg(a.in_u, g);
+      const D2 pu = ldg(a.in_pu, g);
+      rphi[q] = pu.a;
+      ru[q] = pu.b;
       rv[q] = ldg(a.in_v, g);
       const D2 ga = ldg(mpk, 2 * off), gb = ldg(mpk, 2 * off + 1);
       rsg[q] = ga.a;
@@ -228,8 +229,9 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       rrs[q] = sr.b;
       rf[q] = ldg(m.f, g);
       if (need_base) {
-        rbp[q] = ldg(a.b_phi, g);
-        rbu[q] = ldg(a.b_u, g);
+        const D2 bpu = ldg(a.b_pu, g);
+        rbp[q] = bpu.a;
+        rbu[q] = bpu.b;
         rbv[q] = ldg(a.b_v, g);
       }
       if (need_acc) {
@@ -296,8 +298,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
 
       if (kind == kTend) {
         const uint32_t ti = ((uint32_t)(p - 1) * npv + (j - 1)) * npv + (i - 1);
-        stg(a.o_phi, ti, kpg);
-        stg(a.o_u, ti, ku);
+        stg(a.o_pu, ti, D2{kpg, ku});
         stg(a.o_v, ti, kv);
       } else {
         double pgn, un, vn;  // base state Y_n
@@ -334,8 +335,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
           v = div_by(vn, k3) + 2.0 / 3.0 * sV[li] + a.c1 * kv;
         }
         const double phi = div_by(pg, sgr);  // time_scheme_mod.F90:81
-        stg(a.o_phi, g, phi);
-        stg(a.o_u, g, u);
+        stg(a.o_pu, g, D2{phi, u});
         stg(a.o_v, g, v);
         // raw copy of the lines nearest each side for the boundary pass
         if (i <= H + 1 || i >= npv - H || j <= H + 1 || j >= npv - H) {
