@@ -36,12 +36,13 @@ struct MeshF32 {
 };
 
 struct StageArgs32 {
-  const float *in_phi, *in_u, *in_v;       // stage state (phi', u, v)
-  const float *b_phi, *b_u, *b_v;          // base Y_n (hi)
-  const float *c_phi, *c_u, *c_v;          // base Y_n compensation (lo)
-  float *acc_phiG, *acc_u, *acc_v;         // RK4 accumulator
-  float *o_phi, *o_u, *o_v;                // stage output (raw interior)
-  float *oc_phi, *oc_u, *oc_v;             // compensation of the output (last stage only)
+  // each level, compensation and accumulator as {phi', u} pairs and v
+  const F2 *in_pu; const float *in_v;      // stage state (phi', u, v)
+  const F2 *b_pu; const float *b_v;        // base Y_n (hi)
+  const F2 *c_pu; const float *c_v;        // base Y_n compensation (lo)
+  F2 *acc_pu; float *acc_v;                // RK4 accumulator
+  F2 *o_pu; float *o_v;                    // stage output (raw interior)
+  F2 *oc_pu; float *oc_v;                  // compensation of the output (last stage only)
   float *band_raw;                         // raw band lines (hi), see band_slot()
   float *band_raw_c;                       // raw band lines (lo), last stage only
   double c1;                               // stage increment coefficient (x dt)
@@ -130,8 +131,9 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       if (!live[q]) continue;
       const uint32_t off = (uint32_t)((j0 - 1 + ly) * nh + (i0 - 1 + lx));
       const uint32_t g = pbase + off;
-      rdp[q] = ldg(a.in_phi, g);
-      ru[q] = ldg(a.in_u, g);
+      const F2 pu = ldg(a.in_pu, g);
+      rdp[q] = pu.a;
+      ru[q] = pu.b;
       rv[q] = ldg(a.in_v, g);
       rpg[q] = ldg(m.pgref, g);
       const D2 ga = ldg(mpk, 2 * off), gb = ldg(mpk, 2 * off + 1), sr = ldg(spk, off);
@@ -268,19 +270,22 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       rdy_[q] = de.b;
       if (kind == kTend) continue;
       if (last) {
-        rcp[q] = ldg(a.c_phi, g);
-        rcu[q] = ldg(a.c_u, g);
+        const F2 cpu = ldg(a.c_pu, g);
+        rcp[q] = cpu.a;
+        rcu[q] = cpu.b;
         rcv[q] = ldg(a.c_v, g);
       }
-      if (need_in) rin[q] = ldg(a.in_phi, g);
+      if (need_in) rin[q] = ldg(a.in_pu, g).a;
       if (!first) {
-        rbp[q] = ldg(a.b_phi, g);
-        rbu[q] = ldg(a.b_u, g);
+        const F2 bpu = ldg(a.b_pu, g);
+        rbp[q] = bpu.a;
+        rbu[q] = bpu.b;
         rbv[q] = ldg(a.b_v, g);
       }
       if (need_acc) {
-        rap[q] = ldg(a.acc_phiG, g);
-        rau[q] = ldg(a.acc_u, g);
+        const F2 apu = ldg(a.acc_pu, g);
+        rap[q] = apu.a;
+        rau[q] = apu.b;
         rav[q] = ldg(a.acc_v, g);
       }
     }
@@ -340,8 +345,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       const double kph = kpg * rsgv;  // d(phi')/dt = d(phiG)/dt / sqrtG
       if (kind == kTend) {
         const uint32_t ti = ((uint32_t)(p - 1) * npv + (j - 1)) * npv + (i - 1);
-        stg(a.o_phi, ti, (float)kpg);
-        stg(a.o_u, ti, (float)ku);
+        stg(a.o_pu, ti, F2{(float)kpg, (float)ku});
         stg(a.o_v, ti, (float)kv);
       } else {
         // base state Y_n: hi (+ lo in the last stage) in FP64
@@ -363,11 +367,12 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         double ph, u, v;
         if (kind == kRK4S1) {
           ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-          stg(a.acc_phiG, g, (float)kph); stg(a.acc_u, g, (float)ku); stg(a.acc_v, g, (float)kv);
+          stg(a.acc_pu, g, F2{(float)kph, (float)ku});
+          stg(a.acc_v, g, (float)kv);
         } else if (kind == kRK4S2 || kind == kRK4S3) {
           ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-          stg(a.acc_phiG, g, (float)((double)rap[q] + 2.0 * kph));
-          stg(a.acc_u, g, (float)((double)rau[q] + 2.0 * ku));
+          stg(a.acc_pu, g,
+              F2{(float)((double)rap[q] + 2.0 * kph), (float)((double)rau[q] + 2.0 * ku)});
           stg(a.acc_v, g, (float)((double)rav[q] + 2.0 * kv));
         } else if (kind == kRK4S4) {
           ph = bph + a.c1 * (((double)rap[q] + kph) * (1.0 / 6.0));
@@ -385,19 +390,17 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           v = bv + (2.0 / 3.0) * ((double)sV[li] - bv) + a.c1 * kv;
         }
         const float hph = (float)ph, hu = (float)u, hv = (float)v;
-        stg(a.o_phi, g, hph);
-        stg(a.o_u, g, hu);
+        stg(a.o_pu, g, F2{hph, hu});
         stg(a.o_v, g, hv);
-        if (a.oc_phi) {  // last stage: keep the rounding residue (Kahan word)
-          stg(a.oc_phi, g, (float)(ph - (double)hph));
-          stg(a.oc_u, g, (float)(u - (double)hu));
+        if (a.oc_pu) {  // last stage: keep the rounding residue (Kahan word)
+          stg(a.oc_pu, g, F2{(float)(ph - (double)hph), (float)(u - (double)hu)});
           stg(a.oc_v, g, (float)(v - (double)hv));
         }
         // raw band lines (and their residues in the last stage) for the boundary pass
         if (i <= H + 1 || i >= npv - H || j <= H + 1 || j >= npv - H) {
           const float vals[3] = {hph, hu, hv};
           float lows[3] = {0.0f, 0.0f, 0.0f};
-          if (a.oc_phi) {
+          if (a.oc_pu) {
             lows[0] = (float)(ph - (double)hph);
             lows[1] = (float)(u - (double)hu);
             lows[2] = (float)(v - (double)hv);
@@ -412,7 +415,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
             if (j >= npv - H) sl[ns++] = band_slot(npv, H, p, 3, npv + 1 - j, fl, i);
             for (int e = 0; e < ns; ++e) {
               stg(a.band_raw, sl[e], vals[fl]);
-              if (a.oc_phi) stg(a.band_raw_c, sl[e], lows[fl]);
+              if (a.oc_pu) stg(a.band_raw_c, sl[e], lows[fl]);
             }
           }
         }
@@ -434,10 +437,12 @@ __global__ __launch_bounds__(kStageThreads) void stage_kernel_f32(MeshF32 m, Sta
 // kernel, values in FP32 storage, arithmetic and ghost weights in FP64.  The
 // deviation phi' is interpolated and averaged like phi (both operations are
 // linear and phi_ref is already continuous across edges).  For the base state
-// (cphi != NULL) values are hi + lo and every written point gets its residue.
+// (cpu != NULL) values are hi + lo and every written point gets its residue.
 struct BoundaryArgs32 {
-  float *phi, *u, *v;      // level to finish
-  float *cphi, *cu, *cv;   // its compensation words (base state only, else NULL)
+  F2 *pu;                  // level to finish: {phi', u} pairs and v
+  float *v;
+  F2 *cpu;                 // its compensation words (base state only, else NULL)
+  float *cv;
   const float *band_raw, *band_raw_c;  // raw band lines; residues for the base state
 };
 
@@ -445,8 +450,8 @@ template <int DOF>
 __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const BoundaryArgs32 &ba,
                                                   const int t) {
   constexpr int H = DOF - 1;
-  float *const phi = ba.phi, *const u = ba.u, *const v = ba.v;
-  float *const cphi = ba.cphi, *const cu = ba.cu, *const cv = ba.cv;
+  F2 *const pu = ba.pu, *const cpu = ba.cpu;
+  float *const v = ba.v, *const cv = ba.cv;
   const int npv = m.npv, nh = m.nh;
   const double dx = m.dx;
   const int nghost = 6 * 4 * H * npv, nedge = 12 * npv;
@@ -456,7 +461,7 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
     const uint32_t s0 = band_slot(npv, H, q, side, d, 0, al), s1 = band_slot(npv, H, q, side, d, 1, al),
                    s2 = band_slot(npv, H, q, side, d, 2, al);
     ph = ldg(ba.band_raw, s0); uu = ldg(ba.band_raw, s1); vv = ldg(ba.band_raw, s2);
-    if (cphi) {
+    if (cpu) {
       ph += (double)ldg(ba.band_raw_c, s0);
       uu += (double)ldg(ba.band_raw_c, s1);
       vv += (double)ldg(ba.band_raw_c, s2);
@@ -464,10 +469,10 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
   };
   auto store = [&](long long g, double ph, double uu, double vv) {
     const float a0 = (float)ph, a1 = (float)uu, a2 = (float)vv;
-    phi[g] = a0; u[g] = a1; v[g] = a2;
-    if (cphi) {
-      cphi[g] = (float)(ph - (double)a0);
-      cu[g] = (float)(uu - (double)a1);
+    pu[g] = F2{a0, a1};
+    v[g] = a2;
+    if (cpu) {
+      cpu[g] = F2{(float)(ph - (double)a0), (float)(uu - (double)a1)};
       cv[g] = (float)(vv - (double)a2);
     }
   };
@@ -630,32 +635,30 @@ __global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, F2 *dE) {
 // (zero in the interior, the halo-and-edge correction elsewhere), u and v,
 // each rounded to FP32 with its residue as the compensation word.
 static __global__ void split_state_kernel(long long np, const double *phi, const double *phi_ref,
-                                   const double *u, const double *v, float *dphi, float *fu,
-                                   float *fv, float *cphi, float *cu, float *cv) {
+                                   const double *u, const double *v, F2 *hpu, float *hv,
+                                   F2 *cpu, float *cv) {
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
        g += (long long)gridDim.x * blockDim.x) {
     const double d = phi[g] - phi_ref[g];
-    const float hd = (float)d, hu = (float)u[g], hv = (float)v[g];
-    dphi[g] = hd;
-    cphi[g] = (float)(d - (double)hd);
-    fu[g] = hu;
-    fv[g] = hv;
-    cu[g] = (float)(u[g] - (double)hu);
-    cv[g] = (float)(v[g] - (double)hv);
+    const float hd = (float)d, hu = (float)u[g], hvv = (float)v[g];
+    hpu[g] = F2{hd, hu};
+    hv[g] = hvv;
+    cpu[g] = F2{(float)(d - (double)hd), (float)(u[g] - (double)hu)};
+    cv[g] = (float)(v[g] - (double)hvv);
   }
 }
 
 // Reconstructs the FP64 state of an FP32C level: phi = phi_ref + hi + lo,
 // u, v = hi + lo.
-static __global__ void join_state_kernel(long long np, const double *phi_ref, const float *dphi,
-                                  const float *fu, const float *fv, const float *cphi,
-                                  const float *cu, const float *cv, double *phi, double *u,
-                                  double *v) {
+static __global__ void join_state_kernel(long long np, const double *phi_ref, const F2 *hpu,
+                                  const float *hv, const F2 *cpu, const float *cv, double *phi,
+                                  double *u, double *v) {
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
        g += (long long)gridDim.x * blockDim.x) {
-    phi[g] = phi_ref[g] + ((double)dphi[g] + (double)cphi[g]);
-    u[g] = (double)fu[g] + (double)cu[g];
-    v[g] = (double)fv[g] + (double)cv[g];
+    const F2 h = hpu[g], c = cpu[g];
+    phi[g] = phi_ref[g] + ((double)h.a + (double)c.a);
+    u[g] = (double)h.b + (double)c.b;
+    v[g] = (double)hv[g] + (double)cv[g];
   }
 }
 

@@ -98,9 +98,13 @@ struct mcv_handle {
   // compensation words of Y_n, FP32 accumulator and edge copies; phi_ref is
   // lv[0][0] (the projected initial geopotential, FP64).
   bool fp32c = false;
-  float *lv32[3][3] = {};
-  float *comp32[3] = {};
-  float *acc32[3] = {};
+  // levels, compensation words of Y_n and accumulator: {phi', u} pairs and v
+  mcv::F2 *l32pu[3] = {};
+  float *l32v[3] = {};
+  mcv::F2 *c32pu = nullptr;
+  float *c32v = nullptr;
+  mcv::F2 *a32pu = nullptr;
+  float *a32v = nullptr;
   float *band32 = nullptr, *band32c = nullptr;  // FP32 band lines and residues
   double *join[3] = {};  // FP64 reconstruction of Y_n for read-back and integrals
   mcv::MeshF32 mesh32{};

@@ -72,13 +72,13 @@ BoundaryArgs boundary_args64(const mcv_handle *h, int level) {
 
 StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1) {
   StageArgs32 a{};
-  a.in_phi = h->lv32[in][0]; a.in_u = h->lv32[in][1]; a.in_v = h->lv32[in][2];
-  a.b_phi = h->lv32[0][0]; a.b_u = h->lv32[0][1]; a.b_v = h->lv32[0][2];
-  a.c_phi = h->comp32[0]; a.c_u = h->comp32[1]; a.c_v = h->comp32[2];
-  a.acc_phiG = h->acc32[0]; a.acc_u = h->acc32[1]; a.acc_v = h->acc32[2];
-  a.o_phi = h->lv32[out][0]; a.o_u = h->lv32[out][1]; a.o_v = h->lv32[out][2];
+  a.in_pu = h->l32pu[in]; a.in_v = h->l32v[in];
+  a.b_pu = h->l32pu[0]; a.b_v = h->l32v[0];
+  a.c_pu = h->c32pu; a.c_v = h->c32v;
+  a.acc_pu = h->a32pu; a.acc_v = h->a32v;
+  a.o_pu = h->l32pu[out]; a.o_v = h->l32v[out];
   if (out == 0) {  // the step's last stage writes Y_n+1 with its compensation
-    a.oc_phi = h->comp32[0]; a.oc_u = h->comp32[1]; a.oc_v = h->comp32[2];
+    a.oc_pu = h->c32pu; a.oc_v = h->c32v;
   }
   a.band_raw = h->band32;
   a.band_raw_c = h->band32c;
@@ -89,8 +89,8 @@ StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1) {
 BoundaryArgs32 boundary_args32(const mcv_handle *h, int level) {
   const bool base = level == 0;  // Y_n carries its compensation words
   BoundaryArgs32 b{};
-  b.phi = h->lv32[level][0]; b.u = h->lv32[level][1]; b.v = h->lv32[level][2];
-  if (base) { b.cphi = h->comp32[0]; b.cu = h->comp32[1]; b.cv = h->comp32[2]; }
+  b.pu = h->l32pu[level]; b.v = h->l32v[level];
+  if (base) { b.cpu = h->c32pu; b.cv = h->c32v; }
   b.band_raw = h->band32;
   b.band_raw_c = h->band32c;
   return b;
@@ -324,8 +324,8 @@ int materialize(mcv_handle *h, const double **out) {
     return MCV_OK;
   }
   hipLaunchKernelGGL(join_state_kernel, dim3(1024), dim3(256), 0, h->stream, h->np, h->phi_ref,
-                     h->lv32[0][0], h->lv32[0][1], h->lv32[0][2], h->comp32[0], h->comp32[1],
-                     h->comp32[2], h->join[0], h->join[1], h->join[2]);
+                     h->l32pu[0], h->l32v[0], h->c32pu, h->c32v, h->join[0], h->join[1],
+                     h->join[2]);
   HIP_TRY(hipGetLastError());
   for (int f = 0; f < 3; ++f) out[f] = h->join[f];
   return MCV_OK;
@@ -414,12 +414,11 @@ int setup32(mcv_handle *h) {
                        h->phi_ref, dE);
   HIP_TRY(hipGetLastError());
   for (int l = 0; l < 3; ++l)
-    for (int f = 0; f < 3; ++f)
-      if ((rc = dev_alloc_t(h, &h->lv32[l][f], np))) return rc;
-  for (int f = 0; f < 3; ++f) {
-    if ((rc = dev_alloc_t(h, &h->comp32[f], np))) return rc;
-    if ((rc = dev_alloc_t(h, &h->acc32[f], np))) return rc;
-  }
+    if ((rc = dev_alloc_t(h, &h->l32pu[l], np)) || (rc = dev_alloc_t(h, &h->l32v[l], np)))
+      return rc;
+  if ((rc = dev_alloc_t(h, &h->c32pu, np)) || (rc = dev_alloc_t(h, &h->c32v, np)) ||
+      (rc = dev_alloc_t(h, &h->a32pu, np)) || (rc = dev_alloc_t(h, &h->a32v, np)))
+    return rc;
   if ((rc = dev_alloc_t(h, &h->band32, nband)) || (rc = dev_alloc_t(h, &h->band32c, nband)))
     return rc;
   float *pgref;
@@ -428,13 +427,15 @@ int setup32(mcv_handle *h) {
                      pgref);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(split_state_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->join[0],
-                     h->phi_ref, h->join[1], h->join[2], h->lv32[0][0], h->lv32[0][1],
-                     h->lv32[0][2], h->comp32[0], h->comp32[1], h->comp32[2]);
+                     h->phi_ref, h->join[1], h->join[2], h->l32pu[0], h->l32v[0], h->c32pu,
+                     h->c32v);
   HIP_TRY(hipGetLastError());
-  for (int l = 1; l < 3; ++l)
-    for (int f = 0; f < 3; ++f)
-      HIP_TRY(hipMemcpyAsync(h->lv32[l][f], h->lv32[0][f], np * sizeof(float),
-                             hipMemcpyDeviceToDevice, h->stream));
+  for (int l = 1; l < 3; ++l) {
+    HIP_TRY(hipMemcpyAsync(h->l32pu[l], h->l32pu[0], np * sizeof(F2), hipMemcpyDeviceToDevice,
+                           h->stream));
+    HIP_TRY(hipMemcpyAsync(h->l32v[l], h->l32v[0], np * sizeof(float), hipMemcpyDeviceToDevice,
+                           h->stream));
+  }
   MeshF32 &m = h->mesh32;
   const MeshDev &d = h->mesh;
   m.dof = d.dof;
