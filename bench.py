@@ -167,6 +167,9 @@ def time_history(capi, hostlib, s, m):
     return r
 
 
+TILE = {}  # precision -> fixed tile shape (--tile64/--tile32), else autotuned
+
+
 def run_single(capi, m, st, precision, steps, warmup, history=None):
     """Times `steps` steps after warm-up and autotuning; returns the timing
     record and the final state after warmup + 2 + steps + nt steps."""
@@ -174,7 +177,11 @@ def run_single(capi, m, st, precision, steps, warmup, history=None):
     with capi.Solver(DOF, N, DT, m, st, scheme=SCHEME, precision=precision) as s:
         mass0 = s.sync(fields=())["mass"]
         s.step(warmup)
-        tile, tile_ms = s.autotune(reps=5)
+        if precision in TILE:
+            tile, tile_ms = TILE[precision], None
+            s.set_tile(tile)
+        else:
+            tile, tile_ms = s.autotune(reps=5)
         s.step(2)
         s.sync(fields=(), integrals=False)
         t = time.perf_counter()
@@ -388,6 +395,10 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="batches launch every kernel directly instead of replaying hipGraphs "
                          "(for kernel-trace profiling)")
+    ap.add_argument("--tile64", type=int, default=None,
+                    help="fixed stage-kernel tile shape of the FP64 solve (default: autotuned)")
+    ap.add_argument("--tile32", type=int, default=None,
+                    help="fixed stage-kernel tile shape of the FP32C solve (default: autotuned)")
     args = ap.parse_args()
     global GRAPHS
     GRAPHS = not args.no_graphs
@@ -397,6 +408,11 @@ def main():
         return 2
 
     from mcv_sw_amd import capi, hostlib
+
+    if args.tile64 is not None:
+        TILE[capi.FP64] = args.tile64
+    if args.tile32 is not None:
+        TILE[capi.FP32C] = args.tile32
 
     if args.copy_probe_mib:
         nb = args.copy_probe_mib << 20
