@@ -581,7 +581,7 @@ int mcv_batch_autotune(mcv_batch *b, int32_t reps, int32_t *tile, double *ms_per
   int rc = refresh(b);
   if (rc) return rc;
   // The first stage of every scheme reads Y_n and writes only scratch levels,
-  // the accumulator (set, not accumulated) and the edge copy, so repeating
+  // the accumulator (set, not accumulated) and the band, so repeating
   // it leaves the trajectory unchanged; it is only valid at a step boundary.
   for (int s = 0; s < b->capacity; ++s)
     if (b->slot[s] && b->progress[s].phase != 0)

@@ -95,8 +95,8 @@ struct mcv_handle {
   hipEvent_t ev[3] = {};
   double stage_ms = 0.0, boundary_ms = 0.0;
   // FP32C form (precision == MCV_FP32C): FP32 levels of (phi - phi_ref, u, v),
-  // compensation words of Y_n, FP32 accumulator and edge copies; phi_ref is
-  // lv[0][0] (the projected initial geopotential, FP64).
+  // compensation words of Y_n, FP32 accumulator and band copies; phi_ref is
+  // the projected initial geopotential (FP64).
   bool fp32c = false;
   // levels, compensation words of Y_n and accumulator: {phi', u} pairs and v
   mcv::F2 *l32pu[3] = {};

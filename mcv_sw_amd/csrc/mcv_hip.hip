@@ -920,7 +920,7 @@ int mcv_hip_autotune(mcv_handle *h, int32_t reps, int32_t *tile, double *ms_per_
   HIP_TRY(hipSetDevice(h->cfg.device));
   HIP_TRY(hipStreamSynchronize(h->stream));
   // A representative stage (reads Yn and the accumulator, writes S1 and the
-  // accumulator): S1, the accumulator and the raw edge copy are rewritten by
+  // accumulator): S1, the accumulator and the band are rewritten by
   // the first stage of every step, so timing never changes the trajectory.
   StageArgs a{};
   a.in_pu = h->lpu[0]; a.in_v = h->lvv[0];
