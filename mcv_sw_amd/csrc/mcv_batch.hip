@@ -60,16 +60,42 @@ struct BatchProgress {
   long long steps_done;
 };
 
-// LDS of the batched stage kernel for tile variant V: the largest of its four
-// bodies (FP64 / FP32C x MCV3 / MCV4).
-template <int V>
+// Bodies of the batched stage kernel: (precision, order) pairs.  A kernel
+// instantiation compiles the bodies of its mask only, so that a batch of one
+// kind of solve runs with that body's registers and LDS.
+enum : int { kBody64x3 = 1, kBody64x4 = 2, kBody32x3 = 4, kBody32x4 = 8 };
+constexpr int kBodyMasks[] = {kBody64x3, kBody64x4, kBody32x3, kBody32x4,
+                              kBody64x3 | kBody64x4, kBody32x3 | kBody32x4, 15};
+constexpr int kNumMasks = 7;
+
+inline int body_of(int dof, int prec) {
+  return prec == MCV_FP64 ? (dof == 3 ? kBody64x3 : kBody64x4)
+                          : (dof == 3 ? kBody32x3 : kBody32x4);
+}
+
+// The first compiled mask (fewest bodies first) that holds every body in `need`.
+inline int mask_index(int need) {
+  for (int i = 0; i < kNumMasks; ++i)
+    if ((kBodyMasks[i] & need) == need) return i;
+  return kNumMasks - 1;
+}
+
+// LDS of the batched stage kernel for tile variant V and body mask M: the
+// largest of its bodies.
+template <int V, int M>
 struct BatchSmem {
-  static constexpr size_t a = TileLayout<double, 3, TileSet<3>::cx[V], TileSet<3>::cy[V]>::kBytes;
-  static constexpr size_t b = TileLayout<double, 4, TileSet<4>::cx[V], TileSet<4>::cy[V]>::kBytes;
+  static constexpr size_t a =
+      (M & kBody64x3) ? TileLayout<double, 3, TileSet<3>::cx[V], TileSet<3>::cy[V]>::kBytes : 0;
+  static constexpr size_t b =
+      (M & kBody64x4) ? TileLayout<double, 4, TileSet<4>::cx[V], TileSet<4>::cy[V]>::kBytes : 0;
   static constexpr size_t c =
-      TileLayout<FP32CStencil, 3, TileSet32<3>::cx[V], TileSet32<3>::cy[V]>::kBytes;
+      (M & kBody32x3)
+          ? TileLayout<FP32CStencil, 3, TileSet32<3>::cx[V], TileSet32<3>::cy[V]>::kBytes
+          : 0;
   static constexpr size_t d =
-      TileLayout<FP32CStencil, 4, TileSet32<4>::cx[V], TileSet32<4>::cy[V]>::kBytes;
+      (M & kBody32x4)
+          ? TileLayout<FP32CStencil, 4, TileSet32<4>::cx[V], TileSet32<4>::cy[V]>::kBytes
+          : 0;
   static constexpr size_t ab = a > b ? a : b, cd = c > d ? c : d;
   static constexpr size_t kBytes = ab > cd ? ab : cd;
 };
@@ -105,11 +131,11 @@ __device__ __forceinline__ int find_entry(const int *prefix, int cnt, int w) {
 // panel, tile); the work is cut into one contiguous chunk per XCD (blocks are
 // dealt round-robin to the 8 XCDs), so the six panel tiles that share metric
 // data, and neighbouring tiles, go through the same L2.
-template <int V>
+template <int V, int M>
 __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
     const BatchSolve *__restrict__ solves, const BatchProgress *__restrict__ prog,
     const int *__restrict__ list, const int *__restrict__ wprefix, const int *__restrict__ meta) {
-  __shared__ __align__(16) char smem[BatchSmem<V>::kBytes];
+  __shared__ __align__(16) char smem[BatchSmem<V, M>::kBytes];
   const int cnt = meta[0];
   if (cnt == 0) return;
   const int total = wprefix[cnt];
@@ -132,30 +158,43 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
     const int p = local % 6 + 1, tile = local / 6;
     const int ph = st.phase, kind = S.kind[ph];
     const int n = S.mesh.n;
+    // the host launches a mask holding every body in the batch
     if (S.prec == MCV_FP64) {
-      StageArgs a = S.a64[ph];
-      if (st.first) { a.cu0 = S.cu0; a.cv0 = S.cv0; }
-      const MeshDev &m = S.mesh;  // read where used (scalar loads), not copied
-      if (S.dof == 3) {
-        const int tx = tiles_x_of<V>(3, MCV_FP64, n);
-        stage_tile<3, TileSet<3>::cx[V], TileSet<3>::cy[V]>(m, a, kind, p, tile % tx, tile / tx,
-                                                             smem);
-      } else {
-        const int tx = tiles_x_of<V>(4, MCV_FP64, n);
-        stage_tile<4, TileSet<4>::cx[V], TileSet<4>::cy[V]>(m, a, kind, p, tile % tx, tile / tx,
-                                                             smem);
+      if constexpr ((M & (kBody64x3 | kBody64x4)) != 0) {
+        StageArgs a = S.a64[ph];
+        if (st.first) { a.cu0 = S.cu0; a.cv0 = S.cv0; }
+        const MeshDev &m = S.mesh;  // read where used (scalar loads), not copied
+        if (S.dof == 3) {
+          if constexpr ((M & kBody64x3) != 0) {
+            const int tx = tiles_x_of<V>(3, MCV_FP64, n);
+            stage_tile<3, TileSet<3>::cx[V], TileSet<3>::cy[V]>(m, a, kind, p, tile % tx,
+                                                                 tile / tx, smem);
+          }
+        } else {
+          if constexpr ((M & kBody64x4) != 0) {
+            const int tx = tiles_x_of<V>(4, MCV_FP64, n);
+            stage_tile<4, TileSet<4>::cx[V], TileSet<4>::cy[V]>(m, a, kind, p, tile % tx,
+                                                                 tile / tx, smem);
+          }
+        }
       }
     } else {
-      const StageArgs32 &a = S.a32[ph];
-      const MeshF32 &m = S.mesh32;
-      if (S.dof == 3) {
-        const int tx = tiles_x_of<V>(3, MCV_FP32C, n);
-        stage_tile_f32<3, TileSet32<3>::cx[V], TileSet32<3>::cy[V]>(m, a, kind, p, tile % tx,
-                                                                    tile / tx, smem);
-      } else {
-        const int tx = tiles_x_of<V>(4, MCV_FP32C, n);
-        stage_tile_f32<4, TileSet32<4>::cx[V], TileSet32<4>::cy[V]>(m, a, kind, p, tile % tx,
-                                                                    tile / tx, smem);
+      if constexpr ((M & (kBody32x3 | kBody32x4)) != 0) {
+        const StageArgs32 &a = S.a32[ph];
+        const MeshF32 &m = S.mesh32;
+        if (S.dof == 3) {
+          if constexpr ((M & kBody32x3) != 0) {
+            const int tx = tiles_x_of<V>(3, MCV_FP32C, n);
+            stage_tile_f32<3, TileSet32<3>::cx[V], TileSet32<3>::cy[V]>(m, a, kind, p, tile % tx,
+                                                                        tile / tx, smem);
+          }
+        } else {
+          if constexpr ((M & kBody32x4) != 0) {
+            const int tx = tiles_x_of<V>(4, MCV_FP32C, n);
+            stage_tile_f32<4, TileSet32<4>::cx[V], TileSet32<4>::cy[V]>(m, a, kind, p, tile % tx,
+                                                                        tile / tx, smem);
+          }
+        }
       }
     }
     __syncthreads();  // LDS is reused by the next work item
@@ -236,11 +275,12 @@ struct mcv_batch {
   std::vector<void *> allocs;
   int variant = 0;
   bool tables_dirty = true;
-  int grid_stage[kTileVariants] = {};
+  int grid_stage[kTileVariants][kNumMasks] = {};
+  int mask = kNumMasks - 1;  // index into kBodyMasks of the launched stage kernel
   int grid_bnd = 0;
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;
-  int exec_variant = -1;
+  int exec_variant = -1, exec_mask = -1;
   hipEvent_t ev[3] = {};
   int64_t stages_run = 0;
 };
@@ -263,6 +303,7 @@ void drop_graph(mcv_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   b->exec = nullptr;
   b->exec_variant = -1;
+  b->exec_mask = -1;
 }
 
 void destroy_batch(mcv_batch *b) {
@@ -278,14 +319,25 @@ void destroy_batch(mcv_batch *b) {
   delete b;
 }
 
-template <int V>
-int grid_for(mcv_batch *b, int cus) {
+template <int V, int I>
+int grid_for_mask(mcv_batch *b, int cus) {
   int per = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_stage_kernel<V>,
-                                                       kStageThreads, 0));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per, batch_stage_kernel<V, kBodyMasks[I]>, kStageThreads, 0));
   if (per < 1) per = 1;
   const int g = cus * per;
-  b->grid_stage[V] = std::max(kXcds, g / kXcds * kXcds);
+  b->grid_stage[V][I] = std::max(kXcds, g / kXcds * kXcds);
+  return MCV_OK;
+}
+
+template <int V>
+int grid_for(mcv_batch *b, int cus) {
+  int rc;
+  if ((rc = grid_for_mask<V, 0>(b, cus)) || (rc = grid_for_mask<V, 1>(b, cus)) ||
+      (rc = grid_for_mask<V, 2>(b, cus)) || (rc = grid_for_mask<V, 3>(b, cus)) ||
+      (rc = grid_for_mask<V, 4>(b, cus)) || (rc = grid_for_mask<V, 5>(b, cus)) ||
+      (rc = grid_for_mask<V, 6>(b, cus)))
+    return rc;
   return MCV_OK;
 }
 
@@ -337,8 +389,10 @@ int stage_work(int v, const BatchSolve &s) {
 // Rebuilds and uploads the list of occupied slots with its work prefix sums.
 int upload_tables(mcv_batch *b) {
   std::vector<int> list, wp(1, 0), bp(1, 0);
+  int need = 0;
   for (int s = 0; s < b->capacity; ++s) {
     if (!b->slot[s]) continue;
+    need |= body_of(b->solve[s].dof, b->solve[s].prec);
     list.push_back(s);
     wp.push_back(wp.back() + stage_work(b->variant, b->solve[s]));
     bp.push_back(bp.back() + (b->solve[s].nbitems + kBndChunk - 1) / kBndChunk);
@@ -355,14 +409,29 @@ int upload_tables(mcv_batch *b) {
   HIP_TRY(hipMemcpyAsync(b->d_meta, meta, sizeof(meta), hipMemcpyHostToDevice, b->stream));
   // the host vectors go out of scope: finish the copies first
   HIP_TRY(hipStreamSynchronize(b->stream));
+  b->mask = mask_index(need);
   b->tables_dirty = false;
   return MCV_OK;
 }
 
+template <int V, int I>
+void launch_stage_vm(mcv_batch *b) {
+  hipLaunchKernelGGL((batch_stage_kernel<V, kBodyMasks[I]>), dim3(b->grid_stage[V][I]),
+                     dim3(kStageThreads), 0, b->stream, b->d_solve, b->d_prog, b->d_list,
+                     b->d_wprefix, b->d_meta);
+}
+
 template <int V>
 void launch_stage_v(mcv_batch *b) {
-  hipLaunchKernelGGL(batch_stage_kernel<V>, dim3(b->grid_stage[V]), dim3(kStageThreads), 0,
-                     b->stream, b->d_solve, b->d_prog, b->d_list, b->d_wprefix, b->d_meta);
+  switch (b->mask) {
+    case 0: launch_stage_vm<V, 0>(b); break;
+    case 1: launch_stage_vm<V, 1>(b); break;
+    case 2: launch_stage_vm<V, 2>(b); break;
+    case 3: launch_stage_vm<V, 3>(b); break;
+    case 4: launch_stage_vm<V, 4>(b); break;
+    case 5: launch_stage_vm<V, 5>(b); break;
+    default: launch_stage_vm<V, 6>(b); break;
+  }
 }
 
 int launch_stage(mcv_batch *b, int v) {
@@ -408,6 +477,7 @@ int capture(mcv_batch *b) {
   if (e2 != hipSuccess)
     return fail(MCV_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e2));
   b->exec_variant = b->variant;
+  b->exec_mask = b->mask;
   return MCV_OK;
 }
 
@@ -530,7 +600,7 @@ int mcv_batch_run(mcv_batch *b, int32_t nstages) {
   if (b->tables_dirty && (rc = upload_tables(b))) return rc;
   int left = nstages;
   if (b->use_graph && left >= kGraphStages) {
-    if (!b->exec || b->exec_variant != b->variant) {
+    if (!b->exec || b->exec_variant != b->variant || b->exec_mask != b->mask) {
       if ((rc = capture(b))) return rc;
     }
     for (; left >= kGraphStages; left -= kGraphStages)
