@@ -85,6 +85,26 @@ def _continuous(lib):
         assert _same(b.sync(sc), _alone(lib, C, ic))
 
 
+def _kind_change(lib):
+    """The stage kernel is compiled per set of solve kinds in the batch: a
+    graph captured for one kind is replaced when another kind joins, and
+    again when it leaves."""
+    B, D = MIX[2], MIX[1]  # FP32C MCV3 RK3, FP64 MCV4 RK3
+    ib, idd = _inputs(B, 40.0), _inputs(D, 60.0)
+    nb, nd = 16, 4
+    with capi.Batch(2, lib=lib) as b:
+        b.set_graphs(True)
+        sb = b.add(B[0], B[1], B[2], *ib, nb, scheme=B[3], precision=B[4])
+        b.run(12)  # one graph replay with the FP32C MCV3 body alone
+        sd = b.add(D[0], D[1], D[2], *idd, nd, scheme=D[3], precision=D[4])
+        b.run(12)  # re-captured with both bodies
+        _drain(b, [sd], chunk=12)
+        assert _same(b.sync(sd), _alone(lib, (*D[:5], nd), idd))
+        b.remove(sd)
+        _drain(b, [sb], chunk=12)  # back to one body
+        assert _same(b.sync(sb), _alone(lib, (*B[:5], nb), ib))
+
+
 # ---------------------------------------------------------------------------
 # CPU: product sources on the emulation shim
 # ---------------------------------------------------------------------------
@@ -101,6 +121,10 @@ def test_emulated_batch_tile_shapes(emu_lib, tile):
 
 def test_emulated_continuous_batching(emu_lib):
     _continuous(emu_lib)
+
+
+def test_emulated_batch_kind_change(emu_lib):
+    _kind_change(emu_lib)
 
 
 def test_emulated_batch_autotune_keeps_results(emu_lib):
@@ -154,6 +178,12 @@ def test_batch_matches_alone(hip_lib, graphs):
 def test_batch_tile_shapes(hip_lib, tile):
     print(f"[batch] tile={tile}", flush=True)
     _mixed_batch(hip_lib, True, tile)
+
+
+@pytest.mark.gpu
+def test_batch_kind_change(hip_lib):
+    print("[batch] kind change with graph replay", flush=True)
+    _kind_change(hip_lib)
 
 
 @pytest.mark.gpu
