@@ -412,3 +412,29 @@ def test_emulated_history_write_rejects_other_mesh(emu_lib, tmp_path):
 @pytest.mark.gpu
 def test_history_write_rejects_other_mesh(hip_lib, tmp_path):
     _check_rejects_other_mesh(hip_lib, tmp_path)
+
+
+def test_emulated_fp32c_history_fields_track_fp64(emu_lib):
+    """The output block on an FP32C solve: the same fields, within the form's
+    distance of the FP64 solve's."""
+    rec, z, m, st = _history_case("history_mcv3_n4_case5_rk4")
+    out = []
+    for prec in (capi.FP64, capi.FP32C):
+        with capi.Solver(rec["dof"], rec["n"], rec["dt"], m, st, scheme=rec["scheme"],
+                         precision=prec, lib=emu_lib) as s:
+            s.step(3)
+            out.append(s.history_fields())
+    a, b = out
+    cm = corner_mask(rec["dof"], rec["n"])
+
+    def part(d, k):
+        return d[k][:, ~cm] if k in capi.HISTORY_POINT_FIELDS else d[k]
+
+    for k in capi.HISTORY_POINT_FIELDS:  # FillValue-derived corners are equal
+        assert np.array_equal(a[k][:, cm], b[k][:, cm]), k
+    # scalars on their own scale, wind components on their vector's
+    for group in (("phi",), ("phi_t",), ("phiC",), ("u", "v"), ("contraU", "contraV"),
+                  ("zonal", "meridional"), ("uC", "vC")):
+        scale = np.sqrt(sum(np.linalg.norm(part(a, k)) ** 2 for k in group))
+        for k in group:
+            assert np.linalg.norm(part(b, k) - part(a, k)) <= 1e-6 * scale, k
