@@ -4,20 +4,22 @@
 // resolution, precision (FP64/FP32C), RK scheme (RK3_TVD/RK4) and time step.
 // Each stage (three launches) advances every listed solve by ONE RK stage:
 //
-//   batch_stage_kernel     persistent; walks the concatenated (solve, panel,
-//                          tile) work list of all listed solves, each at its
-//                          own stage of its own scheme (the stage index lives
-//                          on the device, so solves admitted at different
-//                          times interleave freely)
-//   batch_boundary_kernel  persistent; the boundary pass of every solve's
-//                          freshly written level
+//   batch_stage_kernel     one workgroup per item of the concatenated
+//                          (solve, panel, tile) work list of all listed
+//                          solves, each at its own stage of its own scheme
+//                          (the stage index lives on the device, so solves
+//                          admitted at different times interleave freely)
+//   batch_boundary_kernel  one workgroup per unit of 256 boundary items of
+//                          every solve's freshly written level
 //   batch_advance_kernel   one workgroup; advances every solve's stage index
 //                          and step counters
 //
 // All per-solve data the kernels need sits in device tables (solve
 // descriptors, stage state, work prefix sums), so one captured hipGraph of
-// kGraphStages stages serves every batch composition: admitting or
-// retiring a solve rewrites the tables between replays and never re-captures.
+// kGraphStages stages serves every batch composition whose work list fits its
+// stage grid: admitting or retiring a solve rewrites the tables between
+// replays, and the graph is re-captured only when the list outgrows the grid
+// or the set of solve kinds changes.
 // Solves whose step budget is used up are skipped by the kernels until the
 // host retires them.
 #include <hip/hip_runtime.h>
@@ -127,10 +129,13 @@ __device__ __forceinline__ int find_entry(const int *prefix, int cnt, int w) {
   return lo;
 }
 
-// Persistent stage kernel.  Work item w of the concatenated list is (solve,
-// panel, tile); the work is cut into one contiguous chunk per XCD (blocks are
-// dealt round-robin to the 8 XCDs), so the six panel tiles that share metric
-// data, and neighbouring tiles, go through the same L2.
+// Stage kernel.  Work item w of the concatenated list is (solve, panel, tile);
+// the work is cut into one contiguous chunk per XCD (blocks are dealt
+// round-robin to the 8 XCDs), so the six panel tiles that share metric data,
+// and neighbouring tiles, go through the same L2.  The host launches at least
+// 8 * ceil(items / 8) workgroups, so each takes at most one item and the
+// hardware's dynamic dispatch balances the tiles; a smaller grid would walk
+// the chunk in strides.
 template <int V, int M>
 __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
     const BatchSolve *__restrict__ solves, const BatchProgress *__restrict__ prog,
@@ -201,8 +206,9 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
   }
 }
 
-// Persistent boundary kernel: the boundary pass of every listed solve's
-// freshly written level, in work units of kBndChunk items.
+// Boundary kernel: the boundary pass of every listed solve's freshly written
+// level, in work units of kBndChunk items.  The host launches at least one
+// workgroup per unit; a smaller grid would walk the units in strides.
 __global__ __launch_bounds__(256) void batch_boundary_kernel(
     const BatchSolve *__restrict__ solves, const BatchProgress *__restrict__ prog,
     const int *__restrict__ list, const int *__restrict__ bprefix, const int *__restrict__ meta) {
@@ -275,12 +281,14 @@ struct mcv_batch {
   std::vector<void *> allocs;
   int variant = 0;
   bool tables_dirty = true;
-  int grid_stage[kTileVariants][kNumMasks] = {};
+  int stage_items = 0;  // length of the stage work list
+  int stage_grid = 0;   // workgroups of the stage launch (>= 8 * ceil(items / 8))
+  int bnd_units = 0;    // boundary work units
+  int bnd_grid = 0;     // workgroups of the boundary launch (>= units)
   int mask = kNumMasks - 1;  // index into kBodyMasks of the launched stage kernel
-  int grid_bnd = 0;
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;
-  int exec_variant = -1, exec_mask = -1;
+  int exec_variant = -1, exec_mask = -1, exec_grid = 0, exec_bnd_grid = 0;
   hipEvent_t ev[3] = {};
   int64_t stages_run = 0;
 };
@@ -304,6 +312,8 @@ void drop_graph(mcv_batch *b) {
   b->exec = nullptr;
   b->exec_variant = -1;
   b->exec_mask = -1;
+  b->exec_grid = 0;
+  b->exec_bnd_grid = 0;
 }
 
 void destroy_batch(mcv_batch *b) {
@@ -317,28 +327,6 @@ void destroy_batch(mcv_batch *b) {
   for (void *p : b->allocs) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
-}
-
-template <int V, int I>
-int grid_for_mask(mcv_batch *b, int cus) {
-  int per = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per, batch_stage_kernel<V, kBodyMasks[I]>, kStageThreads, 0));
-  if (per < 1) per = 1;
-  const int g = cus * per;
-  b->grid_stage[V][I] = std::max(kXcds, g / kXcds * kXcds);
-  return MCV_OK;
-}
-
-template <int V>
-int grid_for(mcv_batch *b, int cus) {
-  int rc;
-  if ((rc = grid_for_mask<V, 0>(b, cus)) || (rc = grid_for_mask<V, 1>(b, cus)) ||
-      (rc = grid_for_mask<V, 2>(b, cus)) || (rc = grid_for_mask<V, 3>(b, cus)) ||
-      (rc = grid_for_mask<V, 4>(b, cus)) || (rc = grid_for_mask<V, 5>(b, cus)) ||
-      (rc = grid_for_mask<V, 6>(b, cus)))
-    return rc;
-  return MCV_OK;
 }
 
 BatchSolve describe(const mcv_handle *h) {
@@ -398,6 +386,8 @@ int upload_tables(mcv_batch *b) {
     bp.push_back(bp.back() + (b->solve[s].nbitems + kBndChunk - 1) / kBndChunk);
   }
   const int meta[1] = {(int)list.size()};
+  b->stage_items = wp.back();
+  b->bnd_units = bp.back();
   if (!list.empty()) {
     HIP_TRY(hipMemcpyAsync(b->d_list, list.data(), list.size() * sizeof(int),
                            hipMemcpyHostToDevice, b->stream));
@@ -410,13 +400,20 @@ int upload_tables(mcv_batch *b) {
   // the host vectors go out of scope: finish the copies first
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->mask = mask_index(need);
+  // a graph keeps its grids while the work fits in them and fills at least
+  // half of them
+  auto keep = [&](int have, int fit) {
+    return b->exec && have >= fit && have <= 2 * fit ? have : fit;
+  };
+  b->stage_grid = keep(b->exec_grid, std::max(kXcds, (b->stage_items + kXcds - 1) / kXcds * kXcds));
+  b->bnd_grid = keep(b->exec_bnd_grid, std::max(1, b->bnd_units));
   b->tables_dirty = false;
   return MCV_OK;
 }
 
 template <int V, int I>
 void launch_stage_vm(mcv_batch *b) {
-  hipLaunchKernelGGL((batch_stage_kernel<V, kBodyMasks[I]>), dim3(b->grid_stage[V][I]),
+  hipLaunchKernelGGL((batch_stage_kernel<V, kBodyMasks[I]>), dim3(b->stage_grid),
                      dim3(kStageThreads), 0, b->stream, b->d_solve, b->d_prog, b->d_list,
                      b->d_wprefix, b->d_meta);
 }
@@ -446,7 +443,7 @@ int launch_stage(mcv_batch *b, int v) {
 }
 
 int launch_boundary(mcv_batch *b) {
-  hipLaunchKernelGGL(batch_boundary_kernel, dim3(b->grid_bnd), dim3(256), 0, b->stream,
+  hipLaunchKernelGGL(batch_boundary_kernel, dim3(b->bnd_grid), dim3(256), 0, b->stream,
                      b->d_solve, b->d_prog, b->d_list, b->d_bprefix, b->d_meta);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(256), 0, b->stream, b->d_solve,
@@ -478,6 +475,8 @@ int capture(mcv_batch *b) {
     return fail(MCV_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e2));
   b->exec_variant = b->variant;
   b->exec_mask = b->mask;
+  b->exec_grid = b->stage_grid;
+  b->exec_bnd_grid = b->bnd_grid;
   return MCV_OK;
 }
 
@@ -531,12 +530,6 @@ int mcv_batch_create(int32_t device, int32_t capacity, mcv_batch **out) {
                       hipMemcpyHostToDevice));
     const int zero[4] = {0, 0, 0, 0};
     HIP_TRY(hipMemcpy(b->d_meta, zero, sizeof(zero), hipMemcpyHostToDevice));
-    int cus = 0;
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    if ((rc = grid_for<0>(b, cus)) || (rc = grid_for<1>(b, cus)) || (rc = grid_for<2>(b, cus)) ||
-        (rc = grid_for<3>(b, cus)))
-      return rc;
-    b->grid_bnd = std::max(1, cus * 4);
     return MCV_OK;
   };
   int rc = init();
@@ -600,7 +593,8 @@ int mcv_batch_run(mcv_batch *b, int32_t nstages) {
   if (b->tables_dirty && (rc = upload_tables(b))) return rc;
   int left = nstages;
   if (b->use_graph && left >= kGraphStages) {
-    if (!b->exec || b->exec_variant != b->variant || b->exec_mask != b->mask) {
+    if (!b->exec || b->exec_variant != b->variant || b->exec_mask != b->mask ||
+        b->exec_grid != b->stage_grid || b->exec_bnd_grid != b->bnd_grid) {
       if ((rc = capture(b))) return rc;
     }
     for (; left >= kGraphStages; left -= kGraphStages)
