@@ -7,6 +7,7 @@ exactly where the reference does) and with packed-FP32 instructions disabled.
 """
 import os
 import subprocess
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -27,14 +28,30 @@ HIP_SOURCES = ["mcv_hip.hip", "mcv_batch.hip", "mcv_probe.hip", "mcv_netcdf.cpp"
 
 
 def build_hip(verbose=False):
-    srcs = [os.path.join(CSRC, f) for f in HIP_SOURCES]
-    tmp = HIP_LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
-    p = subprocess.run(cmd, capture_output=True, text=True)
-    if p.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + p.stderr[-6000:])
-    if verbose and p.stderr.strip():
-        print(p.stderr)
+    """Compiles every source to an object in parallel, then links."""
+    inc = ["-I", os.path.join(ROOT, "include")]
+    with tempfile.TemporaryDirectory(prefix="mcv_build_") as tmpdir:
+        objs, procs = [], []
+        for f in HIP_SOURCES:
+            obj = os.path.join(tmpdir, os.path.splitext(f)[0] + ".o")
+            objs.append(obj)
+            cmd = [HIPCC] + HIP_FLAGS + ["-c"] + inc + ["-o", obj, os.path.join(CSRC, f)]
+            procs.append((f, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                              stderr=subprocess.STDOUT, text=True)))
+        failed = []
+        for f, p in procs:
+            out, _ = p.communicate()
+            if p.returncode != 0:
+                failed.append(f"{f}:\n{out[-6000:]}")
+            elif verbose and out.strip():
+                print(out)
+        if failed:
+            raise RuntimeError("hipcc failed:\n" + "\n".join(failed))
+        tmp = HIP_LIB + ".tmp"
+        cmd = [HIPCC] + HIP_FLAGS + ["-o", tmp] + objs
+        p = subprocess.run(cmd, capture_output=True, text=True)
+        if p.returncode != 0:
+            raise RuntimeError("hipcc link failed:\n" + p.stderr[-6000:])
     os.replace(tmp, HIP_LIB)
     return HIP_LIB
 

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/mcv_hip.h"
@@ -62,20 +63,33 @@ struct BatchProgress {
   long long steps_done;
 };
 
-// Bodies of the batched stage kernel: (precision, order) pairs.  A kernel
-// instantiation compiles the bodies of its mask only, so that a batch of one
-// kind of solve runs with that body's registers and LDS.
-enum : int { kBody64x3 = 1, kBody64x4 = 2, kBody32x3 = 4, kBody32x4 = 8 };
-constexpr int kBodyMasks[] = {kBody64x3, kBody64x4, kBody32x3, kBody32x4,
-                              kBody64x3 | kBody64x4, kBody32x3 | kBody32x4, 15};
-constexpr int kNumMasks = 7;
+// Bodies of the batched stage kernel: (precision, order) pairs, and the RK
+// schemes its solves use.  A kernel instantiation compiles the bodies of its
+// mask only, so that a batch of one kind of solve runs with that body's
+// registers and LDS; with one scheme in the mask the compiler also drops the
+// other scheme's stage kinds (the RK4 accumulator paths of an RK3 batch).
+enum : int {
+  kBody64x3 = 1, kBody64x4 = 2, kBody32x3 = 4, kBody32x4 = 8,
+  kSchemeRK4 = 16, kSchemeRK3 = 32, kSchemes = kSchemeRK4 | kSchemeRK3
+};
+// body sets (one body, then the FP64 pair, the FP32C pair, all four), each
+// with RK3 alone, RK4 alone and both
+constexpr int kBodyMasks[] = {
+    1 | kSchemeRK3,  1 | kSchemeRK4,  1 | kSchemes,  2 | kSchemeRK3,  2 | kSchemeRK4,
+    2 | kSchemes,    4 | kSchemeRK3,  4 | kSchemeRK4,  4 | kSchemes,  8 | kSchemeRK3,
+    8 | kSchemeRK4,  8 | kSchemes,    3 | kSchemeRK3,  3 | kSchemeRK4,  3 | kSchemes,
+    12 | kSchemeRK3, 12 | kSchemeRK4, 12 | kSchemes,   15 | kSchemeRK3, 15 | kSchemeRK4,
+    15 | kSchemes};
+constexpr int kNumMasks = sizeof(kBodyMasks) / sizeof(kBodyMasks[0]);
 
-inline int body_of(int dof, int prec) {
-  return prec == MCV_FP64 ? (dof == 3 ? kBody64x3 : kBody64x4)
-                          : (dof == 3 ? kBody32x3 : kBody32x4);
+inline int body_of(int dof, int prec, int nstages) {
+  const int body = prec == MCV_FP64 ? (dof == 3 ? kBody64x3 : kBody64x4)
+                                    : (dof == 3 ? kBody32x3 : kBody32x4);
+  return body | (nstages == 4 ? kSchemeRK4 : kSchemeRK3);
 }
 
-// The first compiled mask (fewest bodies first) that holds every body in `need`.
+// The first mask (fewest bodies, then fewest schemes) that holds every bit of
+// `need`.
 inline int mask_index(int need) {
   for (int i = 0; i < kNumMasks; ++i)
     if ((kBodyMasks[i] & need) == need) return i;
@@ -162,6 +176,9 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
     const int local = w - wprefix[k];
     const int p = local % 6 + 1, tile = local / 6;
     const int ph = st.phase, kind = S.kind[ph];
+    // the host launches a mask holding the scheme of every solve in the batch
+    if constexpr ((M & kSchemeRK4) == 0) __builtin_assume(kind >= kRK3S1 && kind <= kRK3S3);
+    if constexpr ((M & kSchemeRK3) == 0) __builtin_assume(kind >= kRK4S1 && kind <= kRK4S4);
     const int n = S.mesh.n;
     // the host launches a mask holding every body in the batch
     if (S.prec == MCV_FP64) {
@@ -380,7 +397,7 @@ int upload_tables(mcv_batch *b) {
   int need = 0;
   for (int s = 0; s < b->capacity; ++s) {
     if (!b->slot[s]) continue;
-    need |= body_of(b->solve[s].dof, b->solve[s].prec);
+    need |= body_of(b->solve[s].dof, b->solve[s].prec, b->solve[s].nstages);
     list.push_back(s);
     wp.push_back(wp.back() + stage_work(b->variant, b->solve[s]));
     bp.push_back(bp.back() + (b->solve[s].nbitems + kBndChunk - 1) / kBndChunk);
@@ -418,17 +435,16 @@ void launch_stage_vm(mcv_batch *b) {
                      b->d_wprefix, b->d_meta);
 }
 
+template <int V, size_t... I>
+void launch_stage_masks(mcv_batch *b, std::index_sequence<I...>) {
+  using Fn = void (*)(mcv_batch *);
+  static constexpr Fn fns[] = {&launch_stage_vm<V, (int)I>...};
+  fns[b->mask](b);
+}
+
 template <int V>
 void launch_stage_v(mcv_batch *b) {
-  switch (b->mask) {
-    case 0: launch_stage_vm<V, 0>(b); break;
-    case 1: launch_stage_vm<V, 1>(b); break;
-    case 2: launch_stage_vm<V, 2>(b); break;
-    case 3: launch_stage_vm<V, 3>(b); break;
-    case 4: launch_stage_vm<V, 4>(b); break;
-    case 5: launch_stage_vm<V, 5>(b); break;
-    default: launch_stage_vm<V, 6>(b); break;
-  }
+  launch_stage_masks<V>(b, std::make_index_sequence<kNumMasks>{});
 }
 
 int launch_stage(mcv_batch *b, int v) {

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -28,6 +29,15 @@
 #define __launch_bounds__(...)
 #define __align__(n) alignas(n)
 #define __restrict__
+// a compiler assumption in the product is a checked condition here
+#define __builtin_assume(cond)                                                      \
+  do {                                                                              \
+    if (!(cond)) {                                                                  \
+      std::fprintf(stderr, "assumption failed: %s (%s:%d)\n", #cond, __FILE__,       \
+                   __LINE__);                                                       \
+      std::abort();                                                                 \
+    }                                                                               \
+  } while (0)
 
 struct dim3 {
   unsigned x, y, z;

@@ -105,6 +105,31 @@ def _kind_change(lib):
         assert _same(b.sync(sb), _alone(lib, (*B[:5], nb), ib))
 
 
+def _grid_change(lib):
+    """The stage and boundary kernels take one workgroup per work item; a
+    captured graph keeps its grids while the work list fits them and fills at
+    least half.  A larger solve joining a replayed batch outgrows the grids
+    (re-capture), and its leaving shrinks the list below half (re-capture)."""
+    small = (3, 4, 300.0, "RK3_TVD", capi.FP32C, 12)
+    big = (3, 16, 100.0, "RK3_TVD", capi.FP32C, 4)
+    isml, ibig = _inputs(small, 40.0), _inputs(big, 60.0)
+    with capi.Batch(2, lib=lib) as b:
+        b.set_graphs(True)
+        b.set_tile(0)
+        s1 = b.add(small[0], small[1], small[2], *isml, small[5], scheme=small[3],
+                   precision=small[4])
+        b.run(12)  # the small solve's grids
+        s2 = b.add(big[0], big[1], big[2], *ibig, big[5], scheme=big[3], precision=big[4])
+        b.run(12)  # grown
+        _drain(b, [s2], chunk=12)
+        assert _same(b.sync(s2), _alone(lib, big, ibig))
+        b.remove(s2)
+        _drain(b, [s1], chunk=12)  # shrunk
+        _, done = b.poll()
+        assert done[s1] == small[5]
+        assert _same(b.sync(s1), _alone(lib, small, isml))
+
+
 # ---------------------------------------------------------------------------
 # CPU: product sources on the emulation shim
 # ---------------------------------------------------------------------------
@@ -125,6 +150,10 @@ def test_emulated_continuous_batching(emu_lib):
 
 def test_emulated_batch_kind_change(emu_lib):
     _kind_change(emu_lib)
+
+
+def test_emulated_batch_grid_change(emu_lib):
+    _grid_change(emu_lib)
 
 
 def test_emulated_batch_autotune_keeps_results(emu_lib):
@@ -184,6 +213,11 @@ def test_batch_tile_shapes(hip_lib, tile):
 def test_batch_kind_change(hip_lib):
     print("[batch] kind change with graph replay", flush=True)
     _kind_change(hip_lib)
+
+
+@pytest.mark.gpu
+def test_batch_grid_change(hip_lib):
+    _grid_change(hip_lib)
 
 
 @pytest.mark.gpu
