@@ -143,18 +143,18 @@ __device__ __forceinline__ int find_entry(const int *prefix, int cnt, int w) {
   return lo;
 }
 
-// Stage kernel.  Work item w of the concatenated list is (solve, panel, tile);
-// the work is cut into one contiguous chunk per XCD (blocks are dealt
-// round-robin to the 8 XCDs), so the six panel tiles that share metric data,
-// and neighbouring tiles, go through the same L2.  The host launches at least
-// 8 * ceil(items / 8) workgroups, so each takes at most one item and the
+// Body of the stage kernel.  Work item w of the concatenated list is (solve,
+// panel, tile); the work is cut into one contiguous chunk per XCD (blocks are
+// dealt round-robin to the 8 XCDs), so the six panel tiles that share metric
+// data, and neighbouring tiles, go through the same L2.  The host launches at
+// least 8 * ceil(items / 8) workgroups, so each takes at most one item and the
 // hardware's dynamic dispatch balances the tiles; a smaller grid would walk
 // the chunk in strides.
 template <int V, int M>
-__global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
+__device__ __forceinline__ void batch_stage_items(
     const BatchSolve *__restrict__ solves, const BatchProgress *__restrict__ prog,
-    const int *__restrict__ list, const int *__restrict__ wprefix, const int *__restrict__ meta) {
-  __shared__ __align__(16) char smem[BatchSmem<V, M>::kBytes];
+    const int *__restrict__ list, const int *__restrict__ wprefix, const int *__restrict__ meta,
+    char *smem) {
   const int cnt = meta[0];
   if (cnt == 0) return;
   const int total = wprefix[cnt];
@@ -221,6 +221,34 @@ __global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
     }
     __syncthreads();  // LDS is reused by the next work item
   }
+}
+
+// FP32C bodies with one RK scheme at the tile shapes whose LDS allows 8
+// workgroups per CU (8x7 and 10x5 cells for MCV3, 6x4 for MCV4) fit in 64
+// VGPRs; there the kernel is compiled for 8 waves per SIMD, the SGPRs beyond
+// that budget spilling to VGPR lanes (no scratch memory).  Elsewhere the
+// compiler's own choice stands.
+constexpr bool eight_waves(int V, int M) {
+  const int bodies = M & (kBody64x3 | kBody64x4 | kBody32x3 | kBody32x4);
+  if ((bodies & (kBody64x3 | kBody64x4)) != 0 || (M & kSchemes) == kSchemes) return false;
+  return V == 0 || (V == 2 && bodies == kBody32x3);
+}
+
+template <int V, int M>
+__global__ __launch_bounds__(kStageThreads) void batch_stage_kernel(
+    const BatchSolve *__restrict__ solves, const BatchProgress *__restrict__ prog,
+    const int *__restrict__ list, const int *__restrict__ wprefix, const int *__restrict__ meta) {
+  __shared__ __align__(16) char smem[BatchSmem<V, M>::kBytes];
+  batch_stage_items<V, M>(solves, prog, list, wprefix, meta, smem);
+}
+
+template <int V, int M>
+__global__ __launch_bounds__(kStageThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+batch_stage_kernel_8w(const BatchSolve *__restrict__ solves, const BatchProgress *__restrict__ prog,
+                      const int *__restrict__ list, const int *__restrict__ wprefix,
+                      const int *__restrict__ meta) {
+  __shared__ __align__(16) char smem[BatchSmem<V, M>::kBytes];
+  batch_stage_items<V, M>(solves, prog, list, wprefix, meta, smem);
 }
 
 // Boundary kernel: the boundary pass of every listed solve's freshly written
@@ -430,9 +458,15 @@ int upload_tables(mcv_batch *b) {
 
 template <int V, int I>
 void launch_stage_vm(mcv_batch *b) {
-  hipLaunchKernelGGL((batch_stage_kernel<V, kBodyMasks[I]>), dim3(b->stage_grid),
-                     dim3(kStageThreads), 0, b->stream, b->d_solve, b->d_prog, b->d_list,
-                     b->d_wprefix, b->d_meta);
+  if constexpr (eight_waves(V, kBodyMasks[I])) {
+    hipLaunchKernelGGL((batch_stage_kernel_8w<V, kBodyMasks[I]>), dim3(b->stage_grid),
+                       dim3(kStageThreads), 0, b->stream, b->d_solve, b->d_prog, b->d_list,
+                       b->d_wprefix, b->d_meta);
+  } else {
+    hipLaunchKernelGGL((batch_stage_kernel<V, kBodyMasks[I]>), dim3(b->stage_grid),
+                       dim3(kStageThreads), 0, b->stream, b->d_solve, b->d_prog, b->d_list,
+                       b->d_wprefix, b->d_meta);
+  }
 }
 
 template <int V, size_t... I>
