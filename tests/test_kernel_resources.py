@@ -29,13 +29,14 @@ def _field(block, key):
 
 @pytest.fixture(scope="module")
 def kernels():
-    if not os.path.exists(build.HIPCC) and shutil.which("hipcc") is None:
+    hipcc = build.HIPCC if os.path.exists(build.HIPCC) else shutil.which("hipcc")
+    if hipcc is None:
         pytest.skip("hipcc is not installed")
     out = {}
     with tempfile.TemporaryDirectory(prefix="mcv_res_") as tmp:
         procs = []
         for src in SOURCES:
-            cmd = ([build.HIPCC] + [f for f in build.HIP_FLAGS if f not in ("-shared", "-Wall")] +
+            cmd = ([hipcc] + [f for f in build.HIP_FLAGS if f not in ("-shared", "-Wall")] +
                    ["-c", "--offload-device-only", "-Rpass-analysis=kernel-resource-usage",
                     "-I", os.path.join(build.ROOT, "include"),
                     "-o", os.path.join(tmp, src + ".o"), os.path.join(build.CSRC, src)])
