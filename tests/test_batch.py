@@ -130,6 +130,22 @@ def _grid_change(lib):
         assert _same(b.sync(s1), _alone(lib, small, isml))
 
 
+def _one_kind(lib, spec, tile):
+    """A batch of one kind of solve (one body, one RK scheme) runs the stage
+    kernel compiled for that kind alone (for FP32C at the 8x7/6x4 and 10x5
+    shapes, its 8-wave build); two such solves give the bits of each run
+    alone."""
+    inputs = [_inputs(spec, 40.0), _inputs(spec, 70.0)]
+    with capi.Batch(2, lib=lib) as b:
+        b.set_graphs(True)
+        b.set_tile(tile)
+        slots = [b.add(spec[0], spec[1], spec[2], *i, spec[5], scheme=spec[3],
+                       precision=spec[4]) for i in inputs]
+        _drain(b, slots)
+        for s, i in zip(slots, inputs):
+            assert _same(b.sync(s), _alone(lib, spec, i)), f"slot {s} {spec}"
+
+
 # ---------------------------------------------------------------------------
 # CPU: product sources on the emulation shim
 # ---------------------------------------------------------------------------
@@ -150,6 +166,12 @@ def test_emulated_continuous_batching(emu_lib):
 
 def test_emulated_batch_kind_change(emu_lib):
     _kind_change(emu_lib)
+
+
+@pytest.mark.parametrize("tile", [0, 2])
+@pytest.mark.parametrize("k", range(len(MIX)))
+def test_emulated_batch_one_kind(emu_lib, k, tile):
+    _one_kind(emu_lib, MIX[k], tile)
 
 
 def test_emulated_batch_grid_change(emu_lib):
@@ -213,6 +235,13 @@ def test_batch_tile_shapes(hip_lib, tile):
 def test_batch_kind_change(hip_lib):
     print("[batch] kind change with graph replay", flush=True)
     _kind_change(hip_lib)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [0, 2])
+@pytest.mark.parametrize("k", range(len(MIX)))
+def test_batch_one_kind(hip_lib, k, tile):
+    _one_kind(hip_lib, MIX[k], tile)
 
 
 @pytest.mark.gpu
