@@ -172,3 +172,21 @@ def test_rk_step_is_the_reference_composition(dof, scheme):
     msk = cases.masks(dof, n)["defined"]
     for k in ("phi", "phiG", "u", "v", "contraU", "contraV"):
         assert np.array_equal(got[k][:, msk], ref[k][:, msk]), k
+
+
+def test_exact_division_helper_is_ieee_division(tmp_path):
+    """The kernels' division by a known divisor (q = a*y, r = fma(-q, b, a),
+    copysign(fma(r, y, q), q) with y = RN(1/b)) equals IEEE a / b bit for
+    bit, signed zeros included, on the scheme's divisors and random ones."""
+    import os
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is not installed")
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu")
+    exe = str(tmp_path / "div_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math",
+                    f"-I{here}", "-o", exe, os.path.join(here, "div_check.cpp")], check=True)
+    p = subprocess.run([exe], capture_output=True, text=True)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert " 0 mismatches" in p.stdout
