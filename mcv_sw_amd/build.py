@@ -83,19 +83,21 @@ DRIVER = os.path.join(HOST_DIR, "mcv_sw_gpu")
 def build_driver(hip_lib=HIP_LIB, out=DRIVER, verbose=False):
     """The Fortran host program (host/mcv_sw_driver.F90) linked against a
     library exporting the C-ABI (the HIP build by default)."""
-    mod_dir = os.path.join(os.path.dirname(out), "_build_" + os.path.basename(out))
-    os.makedirs(mod_dir, exist_ok=True)
     lib_dir = os.path.dirname(os.path.abspath(hip_lib))
-    cmd = [FC] + FFLAGS + ["-J", mod_dir, "-o", out + ".tmp",
-                           os.path.join(HOST_DIR, "mcv_host_lib.F90"),
-                           os.path.join(HOST_DIR, "mcv_sw_driver.F90"),
-                           os.path.abspath(hip_lib), f"-Wl,-rpath,{lib_dir}"]
-    p = subprocess.run(cmd, capture_output=True, text=True)
+    # module files and the linked program go to files of this process only, so
+    # that concurrent builders (parallel test workers) do not collide
+    tmp = f"{out}.tmp{os.getpid()}"
+    with tempfile.TemporaryDirectory(prefix="mcv_fmod_") as mod_dir:
+        cmd = [FC] + FFLAGS + ["-J", mod_dir, "-o", tmp,
+                               os.path.join(HOST_DIR, "mcv_host_lib.F90"),
+                               os.path.join(HOST_DIR, "mcv_sw_driver.F90"),
+                               os.path.abspath(hip_lib), f"-Wl,-rpath,{lib_dir}"]
+        p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("amdflang failed:\n" + p.stderr[-6000:])
     if verbose and p.stderr.strip():
         print(p.stderr)
-    os.replace(out + ".tmp", out)
+    os.replace(tmp, out)
     return out
 
 

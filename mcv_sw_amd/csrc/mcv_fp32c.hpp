@@ -405,18 +405,17 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
             lows[1] = (float)(u - (double)hu);
             lows[2] = (float)(v - (double)hv);
           }
+          // one store per side the point is near (a corner point is near two)
+          auto put = [&](uint32_t sl, int fl) {
+            stg(a.band_raw, sl, vals[fl]);
+            if (a.oc_pu) stg(a.band_raw_c, sl, lows[fl]);
+          };
 #pragma unroll
           for (int fl = 0; fl < 3; ++fl) {
-            uint32_t sl[4];
-            int ns = 0;
-            if (i <= H + 1) sl[ns++] = band_slot(npv, H, p, 0, i, fl, j);
-            if (i >= npv - H) sl[ns++] = band_slot(npv, H, p, 1, npv + 1 - i, fl, j);
-            if (j <= H + 1) sl[ns++] = band_slot(npv, H, p, 2, j, fl, i);
-            if (j >= npv - H) sl[ns++] = band_slot(npv, H, p, 3, npv + 1 - j, fl, i);
-            for (int e = 0; e < ns; ++e) {
-              stg(a.band_raw, sl[e], vals[fl]);
-              if (a.oc_pu) stg(a.band_raw_c, sl[e], lows[fl]);
-            }
+            if (i <= H + 1) put(band_slot(npv, H, p, 0, i, fl, j), fl);
+            if (i >= npv - H) put(band_slot(npv, H, p, 1, npv + 1 - i, fl, j), fl);
+            if (j <= H + 1) put(band_slot(npv, H, p, 2, j, fl, i), fl);
+            if (j >= npv - H) put(band_slot(npv, H, p, 3, npv + 1 - j, fl, i), fl);
           }
         }
       }
