@@ -142,6 +142,15 @@ int mcv_hip_set_tile(mcv_handle *h, int32_t tile);
  * as the achievable-bandwidth yardstick. */
 int mcv_hip_copy_probe(int32_t device, int64_t bytes, int32_t reps, double *ms_per_copy);
 
+/* Measurement aid (no counterpart in the reference): the streaming-copy
+ * yardstick.  Times copies of `bytes` between two device buffers with
+ * several 16-byte loads in flight per thread (2, 4 or 8) and non-temporal
+ * loads and stores, on 2, 4 or 8 workgroups per CU, reps launches each, and
+ * returns the fastest mean milliseconds per copy and the variant (1..3 for
+ * 2, 4, 8 loads) and workgroups per CU that reached it (either may be NULL). */
+int mcv_hip_stream_probe(int32_t device, int64_t bytes, int32_t reps, double *ms,
+                         int32_t *variant, int32_t *per_cu);
+
 /* Releases the handle and its device memory.  NULL is accepted. */
 int mcv_hip_free(mcv_handle *h);
 

@@ -97,6 +97,10 @@ def load(path=None):
     L.mcv_hip_step_timed.argtypes = [H, ctypes.c_int32, _D, _D]
     L.mcv_hip_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _D]
     L.mcv_hip_copy_probe.restype = ctypes.c_int
+    L.mcv_hip_stream_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _D,
+                                       ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32)]
+    L.mcv_hip_stream_probe.restype = ctypes.c_int
     L.mcv_hip_autotune.argtypes = [H, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _D]
     L.mcv_hip_autotune.restype = ctypes.c_int
     L.mcv_hip_set_tile.argtypes = [H, ctypes.c_int32]
@@ -149,6 +153,20 @@ def copy_probe(nbytes, reps=20, device=0, lib=None):
     if rc != MCV_OK:
         raise MCVError(rc, "mcv_hip_copy_probe failed")
     return ms.value
+
+
+def stream_probe(nbytes, reps=10, device=0, lib=None):
+    """Fastest streaming copy of nbytes (several loads in flight per thread,
+    non-temporal accesses): (ms per copy, loads-in-flight variant, workgroups
+    per CU)."""
+    L = load(lib)
+    ms = ctypes.c_double()
+    v, p = ctypes.c_int32(), ctypes.c_int32()
+    rc = L.mcv_hip_stream_probe(device, int(nbytes), int(reps), ctypes.byref(ms), ctypes.byref(v),
+                                ctypes.byref(p))
+    if rc:
+        raise MCVError(rc, "mcv_hip_stream_probe failed")
+    return ms.value, v.value, p.value
 
 
 def dims(dof, n):

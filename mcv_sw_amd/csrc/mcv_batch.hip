@@ -519,10 +519,13 @@ int capture(mcv_batch *b) {
     return rc;
   }
   if (e != hipSuccess) return fail(MCV_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
-  const hipError_t e2 = hipGraphInstantiateWithFlags(&b->exec, g, 0);
+  // the batch takes the executable graph only once it exists
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e2 = hipGraphInstantiateWithFlags(&ex, g, 0);
   (void)hipGraphDestroy(g);
   if (e2 != hipSuccess)
     return fail(MCV_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e2));
+  b->exec = ex;
   b->exec_variant = b->variant;
   b->exec_mask = b->mask;
   b->exec_grid = b->stage_grid;

@@ -127,6 +127,73 @@ static int run_batch() {
   return 0;
 }
 
+// Graph lifetime across the serving loop's sequence: graph replays with
+// admissions and retirements between them, the work list growing and
+// shrinking (re-capture), tile-shape and graph on/off switches, a new kind of
+// solve joining (a different compiled kernel), autotuning, and teardown with
+// solves still resident.
+static int run_batch_graphs() {
+  struct Spec { int dof, n, scheme, precision; int64_t steps; };
+  const Spec small = {3, 3, MCV_RK3_TVD, MCV_FP32C, 60}, big = {3, 9, MCV_RK3_TVD, MCV_FP32C, 8},
+             other = {4, 4, MCV_RK4, MCV_FP64, 6};
+  std::vector<std::vector<double>> keep;
+  auto add = [&](mcv_batch *b, const Spec &sp, int32_t *slot) -> int {
+    const int H = sp.dof - 1, npv = sp.n * H + 1, nh = npv + 2 * H;
+    const size_t np = (size_t)6 * nh * nh;
+    std::vector<double> sg(np, 1.0), G(4 * np, 0.0), A(4 * np, 0.0), f(np, 1e-4), ps(np, 0.0);
+    std::vector<double> phi(np), phiG(np), u(np, 1.0), v(np, 0.5), coef(H * npv);
+    std::vector<int32_t> iref(H * npv);
+    for (size_t q = 0; q < np; ++q) {
+      G[4 * q] = G[4 * q + 3] = A[4 * q] = A[4 * q + 3] = 1.0;
+      phi[q] = 5e4 + 10.0 * std::cos(3e-3 * (double)(q % 977));
+      phiG[q] = phi[q];
+    }
+    const double dx = (M_PI / 2) / sp.n;
+    for (int q = 0; q < H * npv; ++q) { iref[q] = 1 + (q / H) % sp.n; coef[q] = 0.3 * dx; }
+    mcv_config cfg{MCV_ABI_VERSION, sp.dof, sp.n, sp.scheme, sp.precision, 0, 1e-6, dx};
+    mcv_mesh mesh{sg.data(), G.data(), G.data(), A.data(), A.data(), f.data(), ps.data(),
+                  coef.data(), iref.data()};
+    mcv_state st{phi.data(), phiG.data(), u.data(), v.data(), u.data(), v.data()};
+    return mcv_batch_add(b, &cfg, &mesh, &st, sp.steps, slot);
+  };
+  mcv_batch *b = nullptr;
+  if (mcv_batch_create(0, 4, &b)) return 1;
+  int32_t s0, s1, s2;
+#define MCV_OK_OR_FAIL(x)                                                   \
+  do {                                                                      \
+    if ((x) != MCV_OK) {                                                    \
+      std::printf("%s: %s\n", #x, mcv_hip_last_error());                   \
+      return 1;                                                             \
+    }                                                                       \
+  } while (0)
+  MCV_OK_OR_FAIL(mcv_batch_set_graphs(b, 1));
+  MCV_OK_OR_FAIL(add(b, small, &s0));
+  MCV_OK_OR_FAIL(mcv_batch_run(b, 12));  // captured for the small list
+  MCV_OK_OR_FAIL(add(b, big, &s1));
+  MCV_OK_OR_FAIL(mcv_batch_run(b, 24));  // grown: re-captured
+  MCV_OK_OR_FAIL(mcv_batch_set_tile(b, 2));
+  MCV_OK_OR_FAIL(mcv_batch_run(b, 12));  // another tile shape: re-captured
+  int64_t left[4], done[4];
+  for (int it = 0; it < 20; ++it) {
+    MCV_OK_OR_FAIL(mcv_batch_poll(b, left, done));
+    if (left[s1] == 0) break;
+    MCV_OK_OR_FAIL(mcv_batch_run(b, 12));
+  }
+  MCV_OK_OR_FAIL(mcv_batch_remove(b, s1));
+  MCV_OK_OR_FAIL(mcv_batch_run(b, 12));  // shrunk: re-captured
+  MCV_OK_OR_FAIL(mcv_batch_set_graphs(b, 0));
+  MCV_OK_OR_FAIL(mcv_batch_run(b, 5));
+  MCV_OK_OR_FAIL(mcv_batch_set_graphs(b, 1));
+  MCV_OK_OR_FAIL(mcv_batch_run(b, 1));   // back to a step boundary of the RK3 solve
+  MCV_OK_OR_FAIL(add(b, other, &s2));     // another kind: another kernel, re-captured
+  MCV_OK_OR_FAIL(mcv_batch_run(b, 24));
+  MCV_OK_OR_FAIL(mcv_batch_poll(b, left, done));
+  mcv_batch_free(b);                      // solves still resident
+#undef MCV_OK_OR_FAIL
+  std::printf("batch graphs ok\n");
+  return 0;
+}
+
 int main() {
   setvbuf(stdout, nullptr, _IONBF, 0);
   int rc = 0;
@@ -135,5 +202,6 @@ int main() {
       for (int scheme : {MCV_RK4, MCV_RK3_TVD})
         for (int precision : {MCV_FP64, MCV_FP32C}) rc |= run(dof, n, scheme, precision);
   rc |= run_batch();
+  rc |= run_batch_graphs();
   return rc;
 }

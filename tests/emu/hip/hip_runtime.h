@@ -102,6 +102,9 @@ inline hipError_t hipMemset(void *d, int v, size_t n) {
   std::memset(d, v, n);
   return hipSuccess;
 }
+// Cache hints have no meaning here.
+#define __builtin_nontemporal_load(p) (*(p))
+#define __builtin_nontemporal_store(v, p) (*(p) = (v))
 struct double2 {
   double x, y;
 };

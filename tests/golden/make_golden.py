@@ -67,6 +67,11 @@ CASES = [
      "RK4", 240.0, 360, {}),
     ("mcv4_n16_jet_rk4", 4, 16, ("jet", dict(u0=40.0, h0=8000.0, lmax=16, seed=0, rms_h=60.0)),
      "RK4", 300.0, 24, {}),
+    # one simulated day of each order on the GPU's parity tests
+    ("mcv4_n24_jet_rk4_day", 4, 24, ("jet", dict(u0=40.0, h0=8000.0, lmax=16, seed=0, rms_h=60.0)),
+     "RK4", 120.0, 720, {}),
+    ("mcv3_n32_jet_rk3_day", 3, 32, ("jet", dict(u0=20.0, h0=5960.0, lmax=12, seed=1, rms_h=20.0)),
+     "RK3_TVD", 120.0, 720, {}),
     # the single large solve at its full size (a few steps); its ghost tables
     # are stored too (the reference's host passes its own tables)
     ("mcv4_n384_jet_rk4_full", 4, 384,

@@ -1,10 +1,13 @@
 """The continuous-batching request loop: every request's result is the same
-alone and in the mix (bit for bit, FP64 and FP32C alike), admission respects
+alone and in the mix (bit for bit, FP64 and FP32C alike) and matches the
+oracle (FP64 bit for bit, FP32C within tolerance), admission respects
 the HBM budget and the slot count, and the report is consistent."""
 import numpy as np
 import pytest
 
+import cases
 from mcv_sw_amd import capi, hostlib, serve
+from oracle import oracle
 
 
 def _requests(sizes, hours=(0.5, 2.0), rate=400.0, count=10, seed=3):
@@ -20,7 +23,10 @@ def _alone(lib, r):
     with capi.Solver(r.dof, r.n, r.dt, m, st, scheme=r.scheme, precision=r.precision,
                      lib=lib) as s:
         s.step(r.nsteps)
-        return s.sync(fields=("phi", "u", "v"))
+        out = s.sync(fields=("phi", "u", "v"))
+    # the oracle's FP64 step on the same inputs (time_scheme_mod.F90:13-142)
+    out["oracle"] = oracle.run(r.dof, r.n, 4 if r.scheme == "RK4" else 3, r.dt, m, st, r.nsteps)
+    return out
 
 
 def _check_serve(lib, reqs, **kw):
@@ -35,6 +41,15 @@ def _check_serve(lib, reqs, **kw):
         for k in ("phi", "u", "v"):
             assert np.array_equal(res[r.rid][k], a[k]), (r, k)
         assert res[r.rid]["mass"] == a["mass"], r
+        ref, got = a["oracle"], res[r.rid]
+        if r.precision == capi.FP64:  # bit for bit on every defined point
+            msk = cases.masks(r.dof, r.n)["defined"]
+            for k in ("phi", "u", "v"):
+                assert np.array_equal(got[k][:, msk], ref[k][:, msk]), (r, k)
+        else:  # FP32C within tolerance of the FP64 trajectory
+            msk = cases.masks(r.dof, r.n)["interior"]
+            for k, tol in (("phi", 1e-6), ("u", 1e-5), ("v", 1e-5)):
+                assert cases.rel_l2(got[k][:, msk], ref[k][:, msk]) <= tol, (r, k)
     return rep
 
 
