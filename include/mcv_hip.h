@@ -113,6 +113,20 @@ int mcv_hip_sphere_winds(mcv_handle *h, double *zonal, double *meridional);
 int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG,
                      double *t_u, double *t_v);
 
+/* No counterpart in the reference: mcv_hip_tendency for callers that cannot
+ * allocate memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  The same kernels and results, bit for
+ * bit, but every temporary lives in a fixed scratch array in the library's
+ * static device memory (32 MiB per device, held until the process ends) or in
+ * buffers the handle allocated at mcv_hip_init; the call allocates no device
+ * or host memory.  Meshes whose input pairs plus tendencies exceed the
+ * scratch (about 800k points per field: MCV3 up to n ~ 180, MCV4 up to
+ * n ~ 120) are refused with MCV_EINVAL.  All handles on one device share its
+ * scratch: calls that overlap on one device (from several host threads or
+ * on handles whose streams run concurrently) race.  Blocks. */
+int mcv_hip_tendency_noalloc(mcv_handle *h, const mcv_state *in, double *t_phiG,
+                             double *t_u, double *t_v);
+
 /* Number of completed time steps since mcv_hip_init. */
 int mcv_hip_steps_done(const mcv_handle *h, int64_t *steps);
 

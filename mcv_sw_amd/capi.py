@@ -93,6 +93,7 @@ def load(path=None):
     L.mcv_hip_sync.argtypes = [H, ctypes.POINTER(State), _D, _D]
     L.mcv_hip_sphere_winds.argtypes = [H, _D, _D]
     L.mcv_hip_tendency.argtypes = [H, ctypes.POINTER(State), _D, _D, _D]
+    L.mcv_hip_tendency_noalloc.argtypes = [H, ctypes.POINTER(State), _D, _D, _D]
     L.mcv_hip_steps_done.argtypes = [H, ctypes.POINTER(ctypes.c_int64)]
     L.mcv_hip_step_timed.argtypes = [H, ctypes.c_int32, _D, _D]
     L.mcv_hip_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _D]
@@ -133,7 +134,7 @@ def load(path=None):
     L.mcv_hip_history_fields.argtypes = [H, ctypes.POINTER(HistoryFields)]
     L.mcv_hip_history_write.argtypes = [H, ctypes.c_char_p, ctypes.c_int32]
     for f in ("mcv_hip_init", "mcv_hip_step", "mcv_hip_sync", "mcv_hip_sphere_winds",
-              "mcv_hip_tendency", "mcv_hip_steps_done", "mcv_hip_step_timed", "mcv_hip_free",
+              "mcv_hip_tendency", "mcv_hip_tendency_noalloc", "mcv_hip_steps_done", "mcv_hip_step_timed", "mcv_hip_free",
               "mcv_hip_footprint", "mcv_hip_device_bytes", "mcv_batch_create", "mcv_batch_add",
               "mcv_batch_remove", "mcv_batch_run", "mcv_batch_poll", "mcv_batch_sync",
               "mcv_batch_handle", "mcv_batch_autotune", "mcv_batch_set_tile",
@@ -319,11 +320,14 @@ class Solver:
         """The output block of MCV_SW.F90:62-70: record `time_slot` of `path`."""
         self._check(self.lib.mcv_hip_history_write(self._h, os.fsencode(path), int(time_slot)))
 
-    def tendency(self, state):
+    def tendency(self, state, noalloc=False):
+        """The spatial operator alone on `state`; noalloc: through
+        mcv_hip_tendency_noalloc (library scratch, no allocation in the call)."""
         keep = [np.ascontiguousarray(state[k], dtype=np.float64) for k in STATE_FIELDS]
         st = State(*[_ptr(a) for a in keep])
         out = [np.empty((6, self.npv, self.npv)) for _ in range(3)]
-        self._check(self.lib.mcv_hip_tendency(self._h, ctypes.byref(st), *[_ptr(o) for o in out]))
+        fn = self.lib.mcv_hip_tendency_noalloc if noalloc else self.lib.mcv_hip_tendency
+        self._check(fn(self._h, ctypes.byref(st), *[_ptr(o) for o in out]))
         return dict(zip(("phiG", "u", "v"), out))
 
     def autotune(self, reps=5):

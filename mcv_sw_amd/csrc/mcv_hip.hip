@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -99,6 +100,37 @@ BoundaryArgs32 boundary_args32(const mcv_handle *h, int level) {
 }  // namespace mcv
 
 namespace {
+
+// Scratch of the allocation-free entry point mcv_hip_tendency_noalloc: one
+// copy per device (the code object's own static memory, no hipMalloc), held
+// until the process ends.  Host code never passes `scratch` itself to a
+// kernel or a copy (in host code the name is only a stand-in for the device
+// symbol): it uses the device address that hipGetSymbolAddress returned for
+// the current device, kept in g_scratch.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+char *g_scratch[kMaxDevices] = {};
+
+// Looks up this device's scratch address once per device (the device must be
+// current); called from mcv_hip_init.
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(MCV_EINVAL, "device %d is beyond the %d devices the library supports", device,
+                kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return MCV_OK;
+  void *p = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch[device] = static_cast<char *>(p);
+  return MCV_OK;
+}
+
+char *scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < kMaxDevices ? g_scratch[device] : nullptr;
+}
 
 int dev_alloc(mcv_handle *h, double **out, size_t count) {
   void *p = nullptr;
@@ -521,6 +553,7 @@ int init_impl(const mcv_config *cfg, const mcv_mesh *mesh, const mcv_state *st,
       }
 
   HIP_TRY(hipSetDevice(cfg->device));
+  if ((rc = scratch_setup(cfg->device))) return rc;
   HIP_TRY(hipStreamCreate(&h->stream));
   for (hipEvent_t &e : h->ev) HIP_TRY(hipEventCreate(&e));
 
@@ -892,6 +925,59 @@ int mcv_hip_tendency(mcv_handle *h, const mcv_state *in, double *t_phiG, double 
     h->allocs.pop_back();
     h->device_bytes -= (size_t)c * sizeof(double) + 256;
   }
+  return MCV_OK;
+}
+
+int mcv_hip_tendency_noalloc(mcv_handle *h, const mcv_state *in, double *t_phiG, double *t_u,
+                             double *t_v) {
+  if (!h) return fail(MCV_EINVAL, "handle is NULL");
+  if (!in || !in->phi || !in->u || !in->v || !in->contraU || !in->contraV || !t_phiG || !t_u ||
+      !t_v)
+    return fail(MCV_EINVAL, "state or output array is NULL");
+  if (h->fp32c)
+    return fail(MCV_EINVAL, "the tendency seam evaluates the FP64 operator; this handle is FP32C");
+  const long long np = h->np, nt = 6LL * h->npv * h->npv;
+  // scratch: the input as {phi, u} pairs, the {phiG, u} tendency pairs and
+  // the v tendency, each 256-byte aligned
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t o_in = 0, o_pu = up(np * sizeof(D2)), o_v = o_pu + up(nt * sizeof(D2));
+  const size_t need = o_v + nt * sizeof(double);
+  if (need > kScratchBytes)
+    return fail(MCV_EINVAL,
+                "mcv_hip_tendency_noalloc: this mesh needs %zu bytes of scratch, the library "
+                "holds %zu (at most about %lld points per field); use mcv_hip_tendency",
+                need, kScratchBytes, (long long)(kScratchBytes / 40));
+  HIP_TRY(hipSetDevice(h->cfg.device));
+  char *const sp = scratch_of(h->cfg.device);
+  if (!sp) return fail(MCV_ESTATE, "no scratch address for device %d", h->cfg.device);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  // the input through the handle's own read-back buffers (allocated at init)
+  const size_t b = np * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(h->pack[0], in->phi, b, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->pack[2], in->u, b, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->pack[3], in->v, b, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->pack[4], in->contraU, b, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(h->pack[5], in->contraV, b, hipMemcpyHostToDevice, h->stream));
+  D2 *const s_in = reinterpret_cast<D2 *>(sp + o_in);
+  D2 *const s_pu = reinterpret_cast<D2 *>(sp + o_pu);
+  double *const s_v = reinterpret_cast<double *>(sp + o_v);
+  hipLaunchKernelGGL(pack_pairs_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->pack[0],
+                     h->pack[2], s_in);
+  HIP_TRY(hipGetLastError());
+  StageArgs a{};
+  a.in_pu = s_in; a.in_v = h->pack[3];
+  a.cu0 = h->pack[4]; a.cv0 = h->pack[5];
+  a.o_pu = s_pu; a.o_v = s_v;
+  int rc = stage(h, kTend, a);
+  if (rc) return rc;
+  hipLaunchKernelGGL(unpack_pairs_kernel, dim3(1024), dim3(256), 0, h->stream, nt, s_pu,
+                     h->pack[0], h->pack[1]);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(t_phiG, h->pack[0], nt * sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipMemcpyAsync(t_u, h->pack[1], nt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipMemcpyAsync(t_v, s_v, nt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return MCV_OK;
 }
 

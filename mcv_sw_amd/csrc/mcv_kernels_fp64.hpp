@@ -185,6 +185,13 @@ static __global__ void unpack_pairs_kernel(long long n, const D2 *pu, double *a,
   }
 }
 
+// Plain arrays a, b into {a, b} pairs (the layout of a time level).
+static __global__ void pack_pairs_kernel(long long n, const double *a, const double *b, D2 *pu) {
+  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < n;
+       g += (long long)gridDim.x * blockDim.x)
+    pu[g] = D2{a[g], b[g]};
+}
+
 // FillValue of constants_mod.F90:20.
 constexpr double kFillValue = -9999999999999999.0;
 

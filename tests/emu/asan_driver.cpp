@@ -56,6 +56,14 @@ static int run(int dof, int n, int scheme, int precision) {
   if (mcv_hip_sphere_winds(h, z.data(), m.data())) { std::printf("winds: %s\n", mcv_hip_last_error()); return 1; }
   const int trc = mcv_hip_tendency(h, &st, t.data(), t.data() + 6 * npv * npv, t.data() + 12 * npv * npv);
   if (precision == MCV_FP64 ? trc != MCV_OK : trc != MCV_EINVAL) { std::printf("tend: %d %s\n", trc, mcv_hip_last_error()); return 1; }
+  // the allocation-free seam: same bits through the library's static scratch
+  std::vector<double> t2(t.size());
+  const int nrc = mcv_hip_tendency_noalloc(h, &st, t2.data(), t2.data() + 6 * npv * npv,
+                                           t2.data() + 12 * npv * npv);
+  if (precision == MCV_FP64 ? (nrc != MCV_OK || t2 != t) : nrc != MCV_EINVAL) {
+    std::printf("tend noalloc: %d %s\n", nrc, mcv_hip_last_error());
+    return 1;
+  }
   mcv_hip_free(h);
   std::printf("dof %d n %d scheme %d precision %d ok mass %g\n", dof, n, scheme, precision, mass);
   return 0;

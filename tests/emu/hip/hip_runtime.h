@@ -105,6 +105,12 @@ inline hipError_t hipMemset(void *d, int v, size_t n) {
 // Cache hints have no meaning here.
 #define __builtin_nontemporal_load(p) (*(p))
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
+// A device symbol's address is the host object's own address here.
+#define HIP_SYMBOL(x) reinterpret_cast<const void *>(&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+  *p = const_cast<void *>(sym);
+  return hipSuccess;
+}
 struct double2 {
   double x, y;
 };
