@@ -22,10 +22,12 @@ Also reported on the same line:
                from HIP events on the solver's stream, against 8 TB/s
   batched_uneven  128 solves of mixed order and size (n in 32..256) in one
                batch, FP32C, RK4, CFL time steps; cell-steps/s of all solves
-  serve        the serving loop: Poisson arrivals (20 req/s) of seeded solves
-               of mixed order, size, length and precision, continuous batching
-               with admission by HBM footprint; completed solves/s and p50/p99
-               latency
+  serve        the serving loop: Poisson arrivals of seeded solves of mixed
+               order, size, length and precision (the configured mix) at a
+               rate one GPU sustains (default 0.8 req/s for 60 s), continuous
+               batching with admission by HBM footprint; offered and served
+               cell-steps/s, steady-state p50/p99 latency, and the offered
+               load of the configured 20 req/s
   cpu_baseline the reference Fortran (oracle/_ref, built from the reference's
                own sources) on this host's cores, on a 2-step sample of the
                headline workload
@@ -223,38 +225,52 @@ def fp64_check(capi, b, slots, specs, steps_of):
 
 
 def run_ensemble(capi, hostlib, stages, warmup_stages):
-    """The uniform ensemble in one batch; cell-steps/s of all members."""
+    """The uniform ensemble in one batch, FP32C, and the same 64 members
+    advanced in FP64 in a batch of their own; cell-steps/s of all members."""
+    import concurrent.futures as cf
+
     m = hostlib.mesh(E_DOF, E_N)
     cells = E_MEMBERS * 6 * E_N * E_N
     nst = 3  # RK3 stages per step
     total_steps = (warmup_stages + stages + 2 * stages) // nst + 2
-    with capi.Batch(E_MEMBERS) as b:
-        b.set_graphs(GRAPHS)
-        t0 = time.time()
-        checked = (0, E_MEMBERS - 1)
-        specs = {}
-        for k in range(E_MEMBERS):
-            st = hostlib.jet(m, u0=E_U0, h0=E_H0, lmax=E_LMAX, seed=k + 1, rms_h=E_RMS)
-            slot = b.add(E_DOF, E_N, E_DT, m, st, total_steps, scheme=E_SCHEME,
-                         precision=capi.FP32C)
-            if k in checked:
-                specs[slot] = (E_DOF, E_N, E_DT, m, st, E_SCHEME,
-                               b.sync(slot, fields=())["mass"])
-        log(f"ensemble: {E_MEMBERS} members admitted in {time.time() - t0:.1f} s")
-        tile, tile_ms = b.autotune(reps=3)
-        b.run(warmup_stages)
-        b.poll()
-        t = time.perf_counter()
-        b.run(stages)
-        b.poll()
-        wall = time.perf_counter() - t
-        stage_ms, bnd_ms = b.run_timed(stages)
-        left, done = b.poll()
-        errs = fp64_check(capi, b, sorted(specs), specs, lambda sl: done[sl])
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(16) as pool:
+        states = list(pool.map(lambda k: hostlib.jet(dict(m), u0=E_U0, h0=E_H0, lmax=E_LMAX,
+                                                     seed=k + 1, rms_h=E_RMS),
+                               range(E_MEMBERS)))
+    log(f"ensemble: {E_MEMBERS} initial states in {time.time() - t0:.1f} s")
+
+    def timed(precision):
+        with capi.Batch(E_MEMBERS) as b:
+            b.set_graphs(GRAPHS)
+            checked = (0, E_MEMBERS - 1)
+            specs = {}
+            for k, st in enumerate(states):
+                slot = b.add(E_DOF, E_N, E_DT, m, st, total_steps, scheme=E_SCHEME,
+                             precision=precision)
+                if k in checked:
+                    specs[slot] = (E_DOF, E_N, E_DT, m, st, E_SCHEME,
+                                   b.sync(slot, fields=())["mass"])
+            tile, tile_ms = b.autotune(reps=3)
+            b.run(warmup_stages)
+            b.poll()
+            t = time.perf_counter()
+            b.run(stages)
+            b.poll()
+            wall = time.perf_counter() - t
+            stage_ms, bnd_ms = b.run_timed(stages)
+            left, done = b.poll()
+            errs = (fp64_check(capi, b, sorted(specs), specs, lambda sl: done[sl])
+                    if precision == capi.FP32C else None)
+        return wall, stage_ms, bnd_ms, tile, tile_ms, int(done[:E_MEMBERS].min()), errs
+
+    wall, stage_ms, bnd_ms, tile, tile_ms, dmin, errs = timed(capi.FP32C)
+    wall64, stage64, bnd64, tile64, _, _, _ = timed(capi.FP64)
     steps = stages / nst
     bpps = fp32c_bytes_per_point_step(E_SCHEME)
     pts = E_MEMBERS * 6 * (E_N * (E_DOF - 1) + 1) ** 2
     achieved = bpps * pts * steps / (stage_ms / 1e3) / 1e9
+    ach64 = words_per_point(E_SCHEME) * 8 * pts * steps / (stage64 / 1e3) / 1e9
     return {"workload": f"{E_MEMBERS} members MCV3 6x{E_N}x{E_N}, RK3, dt={E_DT} s, FP32C, "
                         f"one fused launch per stage for all members, hipGraph replay",
             "value": cells * steps / wall, "unit": "cell-steps/s",
@@ -264,8 +280,15 @@ def run_ensemble(capi, hostlib, stages, warmup_stages):
             "stage_kernel_GBs": achieved, "stage_kernel_frac": achieved / PEAK_HBM_GBS,
             "algorithmic_bytes_per_point_step": bpps,
             "tile_variant": tile, "tile_variant_ms_per_stage": tile_ms,
-            "members_done_steps": int(done[:E_MEMBERS].min()),
-            "fp32c_error_vs_fp64_alone": errs}
+            "members_done_steps": dmin,
+            "fp32c_error_vs_fp64_alone": errs,
+            "fp64_same_members": {
+                "value": cells * steps / wall64, "ms_per_step": wall64 * 1e3 / steps,
+                "stage_kernel_ms_per_step": stage64 / steps,
+                "boundary_kernel_ms_per_step": bnd64 / steps,
+                "stage_kernel_GBs": ach64, "stage_kernel_frac": ach64 / PEAK_HBM_GBS,
+                "tile_variant": tile64},
+            "fp32c_speedup_same_members": wall64 / wall}
 
 
 def run_uneven(capi, hostlib, count=128, stages=24, seed=0):
@@ -337,17 +360,32 @@ def run_uneven(capi, hostlib, count=128, stages=24, seed=0):
             "fp32c_error_vs_fp64_alone": errs}
 
 
-def run_serve(capi, seconds, rate=20.0, seed=0, check=6):
-    """The serving workload: Poisson arrivals at `rate` req/s for `seconds`,
-    one seeded solve per request (order, size, length and precision drawn as
-    configured), continuous batching with admission by HBM footprint.  After
-    the run, up to `check` of the smallest FP32C requests are re-run alone in
-    FP64 and their h/u/v distance reported."""
+def run_serve(capi, seconds, rate, seed=0, check=6, warm_frac=0.25):
+    """The serving workload at a steady offered load: Poisson arrivals at
+    `rate` req/s for `seconds`, one seeded solve per request (order, size,
+    length and precision drawn as configured), continuous batching with
+    admission by HBM footprint.  Reports offered against served cell-steps/s
+    and the latency percentiles of the requests that arrived after the first
+    `warm_frac` of the window (the queue's steady part); the configured
+    20 req/s mix is characterised by its offered load only (one GPU serves a
+    small fraction of it).  After the run, up to `check` of the smallest FP32C
+    requests are re-run alone in FP64 and their h/u/v distance reported, and
+    every request that went non-finite is re-run alone in FP64 to tell the
+    scheme's blow-up from the form's."""
     from mcv_sw_amd import hostlib, serve
 
     reqs = serve.draw_requests(rate, seconds, seed=seed)
     srv = serve.Server(capacity=256, hbm_budget=200 << 30, stages_per_poll=12, workers=16)
     rep, res = srv.run(reqs, keep=("phi", "u", "v"))
+    rows = rep["per_request"]
+    by_rid = {r.rid: r for r in reqs}
+    offered = sum(r.cells * r.nsteps for r in reqs) / seconds
+    done_cs = sum(by_rid[x["rid"]].cells * by_rid[x["rid"]].nsteps for x in rows)
+    steady = [x for x in rows if x["arrival"] >= warm_frac * seconds]
+    lat = np.array([x["latency"] for x in steady]) if steady else np.array([np.nan])
+    wait = np.array([x["admit"] - x["arrival"] for x in rows])
+    cfg_reqs = serve.draw_requests(20.0, 600.0, seed=seed)
+    cfg_offered = sum(r.cells * r.nsteps for r in cfg_reqs) / 600.0
     errs = []
     sample = sorted((r for r in reqs if r.precision == capi.FP32C and r.rid in res),
                     key=lambda r: r.cells)[:check]
@@ -361,20 +399,48 @@ def run_serve(capi, seconds, rate=20.0, seed=0, check=6):
         errs.append({"rid": r.rid, "dof": r.dof, "n": r.n, "hours": round(r.hours, 2),
                      "h_rel_l2": e["h_rel_l2"], "u_rel_l2": e["u_rel_l2"],
                      "v_rel_l2": e["v_rel_l2"]})
+    twins = []
+    for x in rows:
+        if x["status"] == "ok":
+            continue
+        r = by_rid[x["rid"]]
+        m = hostlib.mesh(r.dof, r.n)
+        st = hostlib.jet(m, u0=r.u0, h0=r.h0, lmax=r.lmax, seed=r.seed, rms_h=r.rms_h)
+        with capi.Solver(r.dof, r.n, r.dt, m, st, scheme=r.scheme) as s:
+            s.step(r.nsteps)
+            try:
+                s.sync(fields=())
+                fin = True
+            except capi.MCVError:
+                fin = False
+        twins.append({"rid": r.rid, "dof": r.dof, "n": r.n, "hours": round(r.hours, 2),
+                      "precision": r.precision, "fp64_alone_finite": fin})
     sizes = {}
     for r in reqs:
         sizes[f"mcv{r.dof}_n{r.n}"] = sizes.get(f"mcv{r.dof}_n{r.n}", 0) + 1
     out = {k: v for k, v in rep.items() if k != "per_request"}
-    out["non_finite_requests"] = [
-        {k: x[k] for k in ("rid", "dof", "n", "precision", "hours", "u0", "rms_h")}
-        for x in rep["per_request"] if x["status"] != "ok"]
-    out.update({"workload": f"Poisson {rate} req/s for {seconds} s of arrivals; MCV3/MCV4 "
-                            f"p=0.5; n from {{32,64,128,256,384}} w 0.3/0.3/0.2/0.15/0.05; "
-                            f"6-48 simulated hours; FP64/FP32C p=0.5; RK4, CFL dt",
-                "requests_by_size": sizes,
-                "fp32c_error_vs_fp64_alone": errs,
-                "note": "latency = arrival to read-back of the finished solve, including "
-                        "host preparation of the initial state"})
+    out.update({
+        "workload": f"Poisson {rate} req/s for {seconds} s of arrivals; MCV3/MCV4 p=0.5; n from "
+                    f"{{32,64,128,256,384}} w 0.3/0.3/0.2/0.15/0.05; 6-48 simulated hours; "
+                    f"FP64/FP32C p=0.5; RK4, CFL dt",
+        "offered_req_per_s": rate, "window_s": seconds,
+        "offered_cell_steps_per_s": offered,
+        "served_cell_steps_per_s": done_cs / rep["makespan_s"] if rep["makespan_s"] else 0.0,
+        "steady_sample": len(steady),
+        "latency_p50_steady_s": float(np.percentile(lat, 50)),
+        "latency_p99_steady_s": float(np.percentile(lat, 99)),
+        "admission_wait_mean_s": float(wait.mean()) if len(wait) else None,
+        "admission_wait_max_s": float(wait.max()) if len(wait) else None,
+        "configured_20_req_per_s_offered_cell_steps_per_s": cfg_offered,
+        "requests_by_size": sizes,
+        "fp32c_error_vs_fp64_alone": errs,
+        "non_finite_requests": [
+            {k: x[k] for k in ("rid", "dof", "n", "precision", "hours", "u0", "rms_h")}
+            for x in rows if x["status"] != "ok"],
+        "non_finite_fp64_twins": twins,
+        "note": "latency = arrival to read-back of the finished solve, including host "
+                "preparation of the initial state; steady = requests arriving after the "
+                f"first {warm_frac:.0%} of the window"})
     return out
 
 
@@ -388,8 +454,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--ensemble-stages", type=int, default=180,
                     help="timed RK stages of the ensemble (a multiple of 3)")
-    ap.add_argument("--serve-seconds", type=float, default=1.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="seconds of Poisson arrivals for the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=0.8,
+                    help="offered requests per second of the serving run (the configured "
+                         "20 req/s mix offers about 18x what one GPU serves; its offered "
+                         "load is reported beside)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
     ap.add_argument("--no-graphs", action="store_true",
@@ -489,11 +559,22 @@ def main():
         res["batched_uneven"] = run_uneven(capi, hostlib)
         log(f"uneven batch: {res['batched_uneven']['value']:.3e} cell-steps/s")
     if args.serve_seconds > 0:
-        res["serve"] = run_serve(capi, args.serve_seconds)
-        log(f"serve: {res['serve']['completed']} solves, p50 {res['serve']['latency_p50_s']:.2f} s, "
-            f"p99 {res['serve']['latency_p99_s']:.2f} s")
+        res["serve"] = run_serve(capi, args.serve_seconds, args.serve_rate)
+        sv = res["serve"]
+        log(f"serve: {sv['completed']} solves, offered {sv['offered_cell_steps_per_s']:.3e} "
+            f"served {sv['served_cell_steps_per_s']:.3e} cell-steps/s, steady p50 "
+            f"{sv['latency_p50_steady_s']:.2f} s p99 {sv['latency_p99_steady_s']:.2f} s")
     cp_ms = capi.copy_probe(1 << 30, reps=10)
     res["roofline"]["copy_kernel_GBs"] = 2 * (1 << 30) / (cp_ms / 1e3) / 1e9
+    # the streaming-copy yardstick: the best of several loads in flight per
+    # thread and non-temporal accesses, over 1 GiB and 4 GiB
+    best = 0.0
+    for gib in (1, 4):
+        sp_ms, sp_v, sp_pcu = capi.stream_probe(gib << 30, reps=10)
+        gbs = 2 * (gib << 30) / (sp_ms / 1e3) / 1e9
+        res["roofline"][f"stream_copy_{gib}GiB_GBs"] = gbs
+        best = max(best, gbs)
+    res["roofline"]["frac_of_stream_copy"] = achieved / best
     if not args.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline(lambda: hostlib.mesh(DOF, N))

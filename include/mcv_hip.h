@@ -272,10 +272,13 @@ int mcv_history_init(const char *path, const mcv_history_info *info, const doubl
                      const double *areaCell, const double *phi_s);
 
 /* Replaces history_write_stat (output_mod.F90:168-223) for host arrays:
- * writes record `time_slot` (1-based) of an existing history file; the
- * array sizes are the file's dimensions.  Used for the first record, which
- * the reference writes from the initial state (MCV_SW.F90:50). */
-int mcv_history_write(const char *path, int32_t time_slot, const mcv_history_fields *rec);
+ * writes record `time_slot` (1-based) of an existing history file.  The
+ * arrays are those of a dof, n solve (points (nh, nh, 6), cells
+ * (n+2, n+2, 6)); a file whose record variables have other sizes is refused
+ * with MCV_EINVAL.  Used for the first record, which the reference writes
+ * from the initial state (MCV_SW.F90:50). */
+int mcv_history_write(const char *path, int32_t time_slot, int32_t dof, int32_t n,
+                      const mcv_history_fields *rec);
 
 /* calc_VIA (diag_mod.F90:69-93) of the current u, v, phi on the device, FP64,
  * in the reference's summation order: (n+2, n+2, 6) each, halo ring zero.

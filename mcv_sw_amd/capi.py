@@ -128,8 +128,8 @@ def load(path=None):
     L.mcv_batch_run_timed.argtypes = [B, ctypes.c_int32, _D, _D]
     L.mcv_batch_free.argtypes = [B]
     L.mcv_history_init.argtypes = [ctypes.c_char_p, ctypes.POINTER(HistoryInfo)] + [_D] * 6
-    L.mcv_history_write.argtypes = [ctypes.c_char_p, ctypes.c_int32,
-                                    ctypes.POINTER(HistoryFields)]
+    L.mcv_history_write.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.POINTER(HistoryFields)]
     L.mcv_hip_cell_averages.argtypes = [H, _D, _D, _D]
     L.mcv_hip_history_fields.argtypes = [H, ctypes.POINTER(HistoryFields)]
     L.mcv_hip_history_write.argtypes = [H, ctypes.c_char_p, ctypes.c_int32]
@@ -245,7 +245,8 @@ def history_write(path, time_slot, rec, dof, n, lib=None):
     """history_write_stat (output_mod.F90:168-223) of host arrays."""
     L = load(lib)
     f, keep = _history_struct(rec, dims(dof, n)[2], n + 2)
-    rc = L.mcv_history_write(os.fsencode(path), int(time_slot), ctypes.byref(f))
+    rc = L.mcv_history_write(os.fsencode(path), int(time_slot), int(dof), int(n),
+                             ctypes.byref(f))
     if rc != MCV_OK:
         raise MCVError(rc, L.mcv_hip_last_error().decode())
 

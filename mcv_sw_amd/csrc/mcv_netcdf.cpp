@@ -340,8 +340,15 @@ int mcv_history_init(const char *path, const mcv_history_info *info, const doubl
   return rc;
 }
 
-int mcv_history_write(const char *path, int32_t time_slot, const mcv_history_fields *rec) {
-  return mcv::history_write_checked(path, time_slot, rec, -1, -1);
+int mcv_history_write(const char *path, int32_t time_slot, int32_t dof, int32_t n,
+                      const mcv_history_fields *rec) {
+  // the caller's arrays: (nh, nh, 6) points, (n+2, n+2, 6) cells
+  if ((dof != 3 && dof != 4) || n < 1 || n > 4096)
+    return mcv::fail(MCV_EINVAL, "mcv_history_write: dof must be 3 or 4 and n in [1, 4096], got "
+                                 "%d, %d", dof, n);
+  const long long nh = (long long)n * (dof - 1) + 1 + 2 * (dof - 1);
+  return mcv::history_write_checked(path, time_slot, rec, 6 * nh * nh,
+                                    6 * (long long)(n + 2) * (n + 2));
 }
 
 }  // extern "C"

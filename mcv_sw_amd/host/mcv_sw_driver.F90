@@ -78,11 +78,11 @@ module mcv_hip_binding
       real(c_double), intent(in) :: lonC(*), latC(*), lonP(*), latP(*), areaCell(*), phi_s(*)
     end function mcv_history_init
 
-    integer(c_int) function mcv_history_write(path, time_slot, rec) &
+    integer(c_int) function mcv_history_write(path, time_slot, dof, n, rec) &
         bind(C, name='mcv_history_write')
       import :: c_int, c_int32_t, c_char, mcv_history_fields
       character(kind=c_char), intent(in) :: path(*)
-      integer(c_int32_t), value :: time_slot
+      integer(c_int32_t), value :: time_slot, dof, n
       type(mcv_history_fields), intent(in) :: rec
     end function mcv_history_write
 
@@ -257,7 +257,8 @@ program mcv_sw_gpu
     hrec = mcv_history_fields(c_loc(u), c_loc(v), c_loc(cu), c_loc(cv), c_loc(phi), &
                               c_loc(phi_t), c_loc(zonal), c_loc(merid), c_loc(zero_c), &
                               c_loc(zero_c), c_loc(zero_c))
-    call mcv_check(mcv_history_write(trim(history_file)//c_null_char, 1, hrec), &
+    call mcv_check(mcv_history_write(trim(history_file)//c_null_char, 1, int(dof, c_int32_t), &
+                                     int(n, c_int32_t), hrec), &
                    'history_write_stat')
   endif
   call mcv_check(mcv_hip_sync(solve, c_null_ptr, mass0, energy0), 'mcv_hip_sync')

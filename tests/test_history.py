@@ -226,6 +226,24 @@ def test_history_write_rejects_bad_input(tmp_path):
         capi.history_write(str(tmp_path / "missing.nc"), 1, rec, 3, 4, lib=lib)
 
 
+def test_history_write_rejects_a_file_of_another_mesh(tmp_path):
+    """The host arrays of a 6x4x4 MCV3 solve are not written into the file of
+    a 6x8x8 solve (whose records are larger: the write would read past the
+    caller's arrays)."""
+    lib = capi.load()
+    path = str(tmp_path / "big.nc")
+    m = hostlib.mesh(3, 8)
+    capi.history_init(path, 3, 8, 600.0, hostlib.cells(3, 8), m["lonP"], m["latP"], m["phi_s"],
+                      case_num=5, lib=lib)
+    rec = _records(3, 4, 1, np.random.default_rng(1))[0]
+    with pytest.raises(capi.MCVError) as ei:
+        capi.history_write(path, 1, rec, 3, 4, lib=lib)
+    assert ei.value.code == capi.MCV_EINVAL
+    assert "values per record" in str(ei.value)
+    big = _records(3, 8, 1, np.random.default_rng(2))[0]
+    capi.history_write(path, 1, big, 3, 8, lib=lib)
+
+
 @pytest.mark.gpu
 def test_gpu_history_write_round_trip(tmp_path, hip_lib):
     """mcv_hip_history_write produces the records mcv_hip_history_fields
