@@ -1,13 +1,16 @@
 // TEST-ONLY: runs every stage kind of the product library (emulated on the
 // CPU, see hip/hip_runtime.h) under AddressSanitizer on small synthetic
-// problems.  Exit status 0 means no out-of-bounds access was found.
+// problems.  Exit status 0 means no out-of-bounds access was found.  The
+// same driver, built with threaded blocks and ThreadSanitizer (tsan_driver),
+// runs a smaller set ("threads") and fails on any intra-block data race.
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "../../include/mcv_hip.h"
 
-static int run(int dof, int n, int scheme, int precision) {
+static int run(int dof, int n, int scheme, int precision, bool tune = true) {
   const int H = dof - 1, npv = n * H + 1, nh = npv + 2 * H;
   const size_t np = (size_t)6 * nh * nh;
   std::vector<double> sg(np), G(4 * np), IG(4 * np), A(4 * np), IA(4 * np), f(np), ps(np);
@@ -45,7 +48,7 @@ static int run(int dof, int n, int scheme, int precision) {
   if (mcv_hip_init(&cfg, &mesh, &st, &h)) { std::printf("init: %s\n", mcv_hip_last_error()); return 1; }
   int32_t best = -1;
   double tms[MCV_TILE_VARIANTS];
-  if (mcv_hip_autotune(h, 1, &best, tms)) { std::printf("autotune: %s\n", mcv_hip_last_error()); return 1; }
+  if (tune && mcv_hip_autotune(h, 1, &best, tms)) { std::printf("autotune: %s\n", mcv_hip_last_error()); return 1; }
   for (int v = 0; v < MCV_TILE_VARIANTS; ++v) {  // every tile shape through every stage kind
     if (mcv_hip_set_tile(h, v)) return 1;
     if (mcv_hip_step(h, 1)) { std::printf("step: %s\n", mcv_hip_last_error()); return 1; }
@@ -202,9 +205,17 @@ static int run_batch_graphs() {
   return 0;
 }
 
-int main() {
+int main(int argc, char **argv) {
   setvbuf(stdout, nullptr, _IONBF, 0);
   int rc = 0;
+  // "threads": the smaller set the threaded (ThreadSanitizer) build runs
+  if (argc > 1 && std::string(argv[1]) == "threads") {
+    // every stage kind (RK4 in FP64, RK3 in FP32C), each order once, every
+    // tile shape on a partial tile, the boundary pass, diagnostics, a batch
+    rc |= run(3, 4, MCV_RK4, MCV_FP64, false);
+    rc |= run(4, 4, MCV_RK3_TVD, MCV_FP32C, false);
+    return rc | run_batch();
+  }
   for (int dof = 3; dof <= 4; ++dof)
     for (int n : {2, 5, 7, 13})
       for (int scheme : {MCV_RK4, MCV_RK3_TVD})

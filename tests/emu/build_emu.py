@@ -6,6 +6,9 @@ kernels and host orchestration without a GPU.  Two variants:
   libmcv_hip_emu.so   -O2, loaded through ctypes by the parity tests
   asan_driver         -O1 with AddressSanitizer/UBSan: runs every stage kind
                       on a small problem and fails on any out-of-bounds access
+  tsan_driver         -O1, each block run by one host thread per GPU thread
+                      with real barriers, under ThreadSanitizer: fails on any
+                      data race between the threads of a block
 """
 import os
 import subprocess
@@ -43,6 +46,21 @@ def build_lib(force=False):
         subprocess.run(COMMON + ["-O2", "-fPIC", "-shared", "-x", "c++"] + SRCS + ["-o", LIB],
                        check=True)
     return LIB
+
+
+TSAN = os.path.join(OUT, "tsan_driver")
+
+
+def build_tsan(force=False):
+    """Threaded blocks (one host thread per GPU thread, real barriers) under
+    ThreadSanitizer: intra-block data races in the kernels are reported."""
+    os.makedirs(OUT, exist_ok=True)
+    if force or _stale(TSAN, _deps()):
+        cmd = [c if c != "-std=c++17" else "-std=c++20" for c in COMMON]
+        subprocess.run(cmd + ["-O1", "-g", "-DMCV_EMU_THREADS", "-fsanitize=thread", "-pthread",
+                              "-x", "c++"] + SRCS + ["-x", "c++",
+                              os.path.join(HERE, "asan_driver.cpp"), "-o", TSAN], check=True)
+    return TSAN
 
 
 def build_asan(force=False):

@@ -10,6 +10,14 @@
 // single thread (blockDim = 1).  Every kernel in the product loops over its
 // work with a block-strided loop, so one thread covers the whole block and
 // __syncthreads() has nothing to wait for.
+//
+// With MCV_EMU_THREADS defined, a launch instead runs each block with its
+// real blockDim: one host thread per GPU thread (a persistent team),
+// __syncthreads() a barrier among them, __shared__ arrays shared by the team,
+// blocks one after another.
+// Built with ThreadSanitizer (tests/emu/build_emu.py, tsan_driver), this
+// reports any two threads of a block that touch the same LDS or global word
+// without a barrier between them (a missing __syncthreads, an LDS race).
 #pragma once
 
 #include <algorithm>
@@ -19,6 +27,12 @@
 #include <cstring>
 #include <functional>
 #include <vector>
+#ifdef MCV_EMU_THREADS
+#include <barrier>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+#endif
 
 #define __global__
 #define __device__
@@ -44,7 +58,12 @@ struct dim3 {
   dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
 inline thread_local dim3 blockIdx(0), threadIdx(0), blockDim(1), gridDim(1);
+#ifdef MCV_EMU_THREADS
+inline thread_local std::barrier<> *g_emu_block_barrier = nullptr;
+inline void __syncthreads() { g_emu_block_barrier->arrive_and_wait(); }
+#else
 inline void __syncthreads() {}
+#endif
 inline void __threadfence() {}
 inline unsigned atomicAdd(unsigned *p, unsigned v) {
   const unsigned o = *p;
@@ -171,6 +190,87 @@ inline hipError_t hipGraphDestroy(hipGraph_t g) {
   return hipSuccess;
 }
 
+#ifdef MCV_EMU_THREADS
+// A persistent team of host threads, one per GPU thread of a block (created
+// again when a launch asks for another block size).  run() hands every member
+// the same job and returns when all have finished it; the mutex and condition
+// variables order the launches with the team's work.
+struct EmuTeam {
+  std::vector<std::thread> members;
+  unsigned size = 0;
+  std::mutex mu;
+  std::condition_variable start, done;
+  std::function<void(unsigned)> job;
+  unsigned long generation = 0;
+  unsigned pending = 0;
+  bool stop = false;
+  void member(unsigned t) {
+    unsigned long seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(mu);
+      start.wait(lk, [&] { return stop || generation != seen; });
+      if (stop) return;
+      seen = generation;
+      std::function<void(unsigned)> j = job;
+      lk.unlock();
+      j(t);
+      lk.lock();
+      if (--pending == 0) done.notify_one();
+    }
+  }
+  void shutdown() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    start.notify_all();
+    for (auto &m : members) m.join();
+    members.clear();
+    stop = false;
+    size = 0;
+  }
+  void run(unsigned n, std::function<void(unsigned)> f) {
+    if (size != n) {
+      shutdown();
+      size = n;
+      for (unsigned t = 0; t < n; ++t) members.emplace_back([this, t] { member(t); });
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    job = std::move(f);
+    pending = n;
+    ++generation;
+    start.notify_all();
+    done.wait(lk, [&] { return pending == 0; });
+  }
+  ~EmuTeam() { shutdown(); }
+};
+inline EmuTeam g_emu_team;
+
+// Each block is run by the team, one member per GPU thread; the members meet
+// at a barrier after every block.
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)          \
+  do {                                                                       \
+    const dim3 g_ = dim3(grid), b_ = dim3(block);                            \
+    auto op_ = [=]() {                                                       \
+      std::barrier<> bar_(b_.x);                                             \
+      g_emu_team.run(b_.x, [&](unsigned t_) {                                \
+        g_emu_block_barrier = &bar_;                                         \
+        gridDim = g_;                                                        \
+        blockDim = dim3(b_.x);                                               \
+        threadIdx = dim3(t_);                                                \
+        for (unsigned bx_ = 0; bx_ < g_.x; ++bx_) {                          \
+          blockIdx = dim3(bx_);                                              \
+          kernel(__VA_ARGS__);                                               \
+          bar_.arrive_and_wait();                                            \
+        }                                                                    \
+      });                                                                    \
+    };                                                                       \
+    if (g_emu_capture)                                                       \
+      g_emu_capture->ops.push_back(op_);                                     \
+    else                                                                     \
+      op_();                                                                 \
+  } while (0)
+#else
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)          \
   do {                                                                       \
     const dim3 g_ = dim3(grid);                                              \
@@ -188,3 +288,4 @@ inline hipError_t hipGraphDestroy(hipGraph_t g) {
     else                                                                     \
       op_();                                                                 \
   } while (0)
+#endif

@@ -150,6 +150,21 @@ def test_asan_build_of_every_stage_kind():
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-4000:]
 
 
+def test_tsan_build_finds_no_race_within_a_block():
+    """Each block run by one host thread per GPU thread with real barriers,
+    under ThreadSanitizer: no two threads of a block touch the same LDS or
+    global word without a __syncthreads() between them, in any stage kind,
+    tile shape, boundary, diagnostic or batched kernel (a removed barrier in
+    the stage kernel is reported as a data race by this build)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "emu"))
+    import build_emu
+
+    exe = build_emu.build_tsan()
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1 exitcode=66")
+    p = subprocess.run([exe, "threads"], capture_output=True, text=True, timeout=900, env=env)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-6000:]
+
+
 def test_bench_refuses_more_than_one_gpu():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
