@@ -24,7 +24,7 @@ Also reported on the same line:
                batch, FP32C, RK4, CFL time steps; cell-steps/s of all solves
   serve        the serving loop: Poisson arrivals of seeded solves of mixed
                order, size, length and precision (the configured mix) at a
-               rate one GPU sustains (default 0.8 req/s for 60 s), continuous
+               rate one GPU sustains (default 0.5 req/s for 60 s), continuous
                batching with admission by HBM footprint; offered and served
                cell-steps/s, steady-state p50/p99 latency, and the offered
                load of the configured 20 req/s
@@ -456,7 +456,7 @@ def main():
                     help="timed RK stages of the ensemble (a multiple of 3)")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="seconds of Poisson arrivals for the serving run (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=0.8,
+    ap.add_argument("--serve-rate", type=float, default=0.5,
                     help="offered requests per second of the serving run (the configured "
                          "20 req/s mix offers about 18x what one GPU serves; its offered "
                          "load is reported beside)")
