@@ -11,6 +11,10 @@
 // derivatives) runs in FP32 on E' = phi' + K, on the mass flux and on the
 // winds.  The large balanced part of the pressure gradient, the MCV
 // derivative of E_ref = phi_ref + phi_s, is computed once per solve in FP64.
+// The reference mass term phi_ref*sqrtG stays FP64: its FP32 rounding is a
+// fixed error in the mass flux, the same at every step, which the scheme's
+// dissipation does not remove (MCV3 6x32^2 jet, 10 days: 2.7e-6 in h with
+// it rounded to FP32, 1.1e-7 without).
 // The tendencies are assembled and the RK update is done in FP64 (on gfx950
 // plain FP32 and FP64 vector instructions issue at the same rate), then
 // rounded back to FP32 hi/lo.  Metric terms, ghost weights and the
@@ -31,7 +35,7 @@ struct MeshF32 {
   const double *sg, *rsg, *ig11, *ig12, *ig22;  // FP64 metric (same arrays as the FP64 path)
   const D2 *mpack, *spack;      // the same terms packed per point (MeshDev::mpack, spack)
   const float *f;               // RN32(Coriolis parameter)
-  const float *pgref;           // RN32(phi_ref * sqrtG)
+  const double *pgref;          // phi_ref * sqrtG (FP64: a rounded copy is a fixed mass-flux error)
   const F2 *dE;                 // RN32 of the FP64 MCV derivatives (x, y) of phi_ref + phi_s
 };
 
@@ -50,9 +54,11 @@ struct StageArgs32 {
 
 // Arithmetic type of the FP32C stencil (LDS tiles, one-sided fits, Riemann
 // fluxes, interior derivatives); level storage is FP32 whatever this is.
-// FP32: an FP64 stencil on the same FP32 levels moves the distance from the
-// FP64 trajectory by less than 15% (the FP32 rounding of the stored stage
-// states dominates it) and doubles the LDS tile.
+// FP32: an FP64 stencil on the same FP32 levels cuts the distance from the
+// FP64 trajectory by ~15x on stable flows (MCV3 6x32^2 jet, 10 days: 1.1e-7
+// -> 7.5e-9 in h), but doubles the LDS tile, lowers the resident workgroups
+// of the FP32C kernels (8x4 MCV4 tile 6 -> 4, batched MCV3 body 8 -> 5) and
+// costs 3.5% of the step; FP32 already keeps h within 1e-6 there.
 using FP32CStencil = float;
 
 template <typename T>
@@ -119,7 +125,8 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   // loads of all a thread's points first, then the derived values
   constexpr int kP0 = (TX * TY + kStageThreads - 1) / kStageThreads;
   for (int t0 = threadIdx.x; t0 < TX * TY; t0 += kP0 * blockDim.x) {
-    float rdp[kP0], ru[kP0], rv[kP0], rpg[kP0];
+    float rdp[kP0], ru[kP0], rv[kP0];
+    double rpg[kP0];
     double r11[kP0], r12[kP0], r22[kP0], rsg[kP0], rrs[kP0];
     bool live[kP0];
 #pragma unroll
@@ -154,7 +161,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       const double g11 = r11[q], g12 = r12[q], g22 = r22[q];
       const double cud = g11 * ud + g12 * vd, cvd = g12 * ud + g22 * vd;
       const double Kd = 0.5 * (cud * ud + cvd * vd);
-      const C pg = (C)((double)rpg[q] + (double)dp * rsg[q]);
+      const C pg = (C)(rpg[q] + (double)dp * rsg[q]);
       const C cu = (C)cud, cv = (C)cvd;
       sE[li] = (C)((double)dp + Kd);  // E - E_ref
       sU[li] = u;
@@ -563,13 +570,13 @@ static __global__ void round_to_f32_kernel(long long n, const double *x, float *
     y[g] = (float)x[g];
 }
 
-// pgref = RN32(phi_ref * sqrtG) on every point.
-static __global__ void pgref_kernel(MeshDev m, const double *phi_ref, float *pgref) {
+// pgref = phi_ref * sqrtG on every point.
+static __global__ void pgref_kernel(MeshDev m, const double *phi_ref, double *pgref) {
   const long long per = (long long)m.nh * m.nh;
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < 6 * per;
        g += (long long)gridDim.x * blockDim.x) {
     const long long p = g / per;
-    pgref[g] = (float)(phi_ref[g] * m.sqrtG[p * m.metric_pstride + (g - p * per)]);
+    pgref[g] = phi_ref[g] * m.sqrtG[p * m.metric_pstride + (g - p * per)];
   }
 }
 

@@ -453,7 +453,7 @@ int setup32(mcv_handle *h) {
     return rc;
   if ((rc = dev_alloc_t(h, &h->band32, nband)) || (rc = dev_alloc_t(h, &h->band32c, nband)))
     return rc;
-  float *pgref;
+  double *pgref;
   if ((rc = dev_alloc_t(h, &pgref, np))) return rc;
   hipLaunchKernelGGL(pgref_kernel, dim3(1024), dim3(256), 0, h->stream, h->mesh, h->phi_ref,
                      pgref);
