@@ -74,6 +74,20 @@ def bumped_jet(m, u0=20.0, h0=5960.0, amp=30.0):
     return handoff(m, phi, u0 * np.cos(lat), np.zeros_like(phi))
 
 
+def rotated_jet(m, alpha, u0=20.0, h0=5960.0, amp=60.0):
+    """Williamson case 2 with its axis tilted by `alpha` (test_case_mod.F90:52-81
+    at alpha = 0), plus the unbalanced Gaussian height bump of bumped_jet."""
+    lon, lat = m["lonP"], m["latP"]
+    zonal = u0 * (np.cos(lat) * np.cos(alpha) + np.cos(lon) * np.sin(lat) * np.sin(alpha))
+    merid = -u0 * np.sin(lon) * np.sin(alpha)
+    s = -np.cos(lon) * np.cos(lat) * np.sin(alpha) + np.sin(lat) * np.cos(alpha)
+    phi = GRAVITY * h0 - (RADIUS * OMEGA * u0 + u0 ** 2 / 2.0) * s ** 2
+    d2 = (lon - 0.7) ** 2 + (lat - 0.4) ** 2
+    phi = phi + GRAVITY * amp * np.exp(-d2 / 0.05)
+    m["phi_s"] = np.zeros_like(phi)
+    return handoff(m, phi, zonal, merid)
+
+
 def masks(dof, n):
     """Boolean (nh, nh) masks: interior, edge lines, halo strips (corners excluded)."""
     h = dof - 1

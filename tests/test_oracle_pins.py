@@ -10,8 +10,9 @@ Each check fails on a plausible restatement mistake on its own:
     analytic ones at order DOF (a wrong orientation or index is O(1));
   - discrete mass: the w-weighted sum of the phiG tendency telescopes to zero
     up to round-off (diag_mod.F90:10-36 weights);
-  - RK4 / SSP-RK3: temporal self-convergence at order 4 / 3 (a wrong stage
-    coefficient drops the order).
+  - RK4 / SSP-RK3: the local error of one step falls as dt^5 / dt^4 in the
+    panel interiors (a wrong stage weight drops the order; the planted
+    mistake of the negative control does).
 The same pins run on the emulated product code.
 """
 import numpy as np
@@ -118,9 +119,12 @@ def test_tendency_conserves_mass_discretely(dof):
     assert abs(tot) <= 1e-12 * mag
 
 
-def _compose_step(dof, n, m, y, scheme, dt):
+def _compose_step(dof, n, m, y, scheme, dt, planted=False):
     """time_scheme_mod.F90:13-123 composed from the operator and the boundary
-    pass, in numpy (elementwise IEEE arithmetic, no contraction)."""
+    pass, in numpy (elementwise IEEE arithmetic, no contraction).  With
+    `planted`, a plausible weight mistake for the order test's negative
+    control: RK4 final weights (2, 1, 1, 2)/6, SSP-RK3 second stage with
+    (1/2) dt k2 in place of (1/4) dt k2."""
     h, npv, nh = capi.dims(dof, n)
     I = (slice(None), slice(h, h + npv), slice(h, h + npv))
     sg = m["sqrtG"]
@@ -145,11 +149,14 @@ def _compose_step(dof, n, m, y, scheme, dt):
         k3 = oracle.tendency(dof, n, m, y3)
         y4 = update(y, lambda f: y[f][I] + dt * k3[f])
         k4 = oracle.tendency(dof, n, m, y4)
-        t = {f: (k1[f] + 2.0 * k2[f] + 2.0 * k3[f] + k4[f]) / 6.0 for f in ("phiG", "u", "v")}
+        w = (2.0, 1.0, 1.0, 2.0) if planted else (1.0, 2.0, 2.0, 1.0)
+        t = {f: (w[0] * k1[f] + w[1] * k2[f] + w[2] * k3[f] + w[3] * k4[f]) / 6.0
+             for f in ("phiG", "u", "v")}
         return update(y, lambda f: y[f][I] + dt * t[f])
     y1 = update(y, lambda f: y[f][I] + dt * k1[f])
     k2 = oracle.tendency(dof, n, m, y1)
-    y2 = update(y, lambda f: 0.75 * y[f][I] + 0.25 * y1[f][I] + (0.25 * dt) * k2[f])
+    c2 = 0.5 if planted else 0.25
+    y2 = update(y, lambda f: 0.75 * y[f][I] + 0.25 * y1[f][I] + (c2 * dt) * k2[f])
     k3 = oracle.tendency(dof, n, m, y2)
     return update(y, lambda f: y[f][I] / 3.0 + (2.0 / 3.0) * y2[f][I] + ((2.0 / 3.0) * dt) * k3[f])
 
@@ -158,11 +165,12 @@ def _compose_step(dof, n, m, y, scheme, dt):
 @pytest.mark.parametrize("dof", [3, 4])
 def test_rk_step_is_the_reference_composition(dof, scheme):
     """One oracle step equals RK4 / SSP-RK3 built from the separately pinned
-    operator and boundary pass.  (A temporal self-convergence test cannot pin
-    the RK weights here: the upwind term switches on the sign of the
-    contravariant wind, which sits at exactly zero on some grid lines and flips
-    with round-off, so the right-hand side is discontinuous and every RK
-    scheme converges at first order in dt on this operator.)"""
+    operator and boundary pass.  (A global self-convergence test in dt cannot
+    pin the RK weights on this scheme: the boundary pass makes the step's
+    error at the panel edges O(dt^2), and the upwind switch of the cross-wind
+    derivative is discontinuous where the contravariant wind changes sign, so
+    the whole-sphere error converges at first order; the weights are pinned
+    in the panel interiors by test_rk_weights_pinned_by_local_error_order.)"""
     n = 5
     m = hostlib.mesh(dof, n)
     st = hostlib.jet(m, u0=25.0, h0=6000.0, lmax=5, seed=2, rms_h=40.0)
@@ -172,6 +180,54 @@ def test_rk_step_is_the_reference_composition(dof, scheme):
     msk = cases.masks(dof, n)["defined"]
     for k in ("phi", "phiG", "u", "v", "contraU", "contraV"):
         assert np.array_equal(got[k][:, msk], ref[k][:, msk]), k
+
+
+def _interior_local_error(step, dof, n, st, dt, dmin=6):
+    """max over h, u, v of |one step of dt - two steps of dt/2|, relative to
+    the field's maximum, over the points at least `dmin` points away from
+    every panel edge."""
+    h, npv, nh = capi.dims(dof, n)
+    jj, ii = np.meshgrid(np.arange(npv), np.arange(npv), indexing="ij")
+    far = np.minimum(np.minimum(ii, npv - 1 - ii), np.minimum(jj, npv - 1 - jj)) >= dmin
+    a, b = step(st, dt, 1), step(st, 0.5 * dt, 2)
+    return max(np.abs(a[f] - b[f])[:, h:h + npv, h:h + npv][:, far].max() / np.abs(st[f]).max()
+               for f in ("phi", "u", "v"))
+
+
+@pytest.mark.parametrize("scheme", [4, 3])
+@pytest.mark.parametrize("dof", [3, 4])
+def test_rk_weights_pinned_by_local_error_order(dof, scheme):
+    """The RK weights against the mathematics, not a restatement: the local
+    error of one step (one step of dt against two of dt/2) of RK4 / SSP-RK3
+    falls as dt^5 / dt^4, so halving dt divides it by 32 / 16.  Measured in
+    the panel interiors (>= 6 points from every edge) of a rotated jet with a
+    height bump: at the panel edges the reference's boundary pass fills the
+    ghosts from the edge lines before it averages them, so the pass is not a
+    projection and the step's error there is O(dt^2) whatever the weights
+    (time_scheme_mod.F90:136-140).  The flow (rotation 0.7 rad, 60 m bump,
+    6x8x8 mesh, dt 80/40/20 s) keeps the upwind switch of the cross-wind
+    derivative away from its sign change over these steps.  The negative
+    control composes the same step with a plausible weight mistake and must
+    lose the order."""
+    n = 8
+    m = cases.mesh(dof, n)
+    st = oracle.run(dof, n, 4, 1.0, m, cases.rotated_jet(m, 0.7, amp=60.0), 1)
+    want = 2.0 ** (scheme + 1)
+
+    def ref_step(s, dt, k):
+        return oracle.run(dof, n, scheme, dt, m, s, k)
+
+    def planted_step(s, dt, k):
+        for _ in range(k):
+            s = _compose_step(dof, n, m, s, scheme, dt, planted=True)
+        return s
+
+    e = [_interior_local_error(ref_step, dof, n, st, dt) for dt in (80.0, 40.0, 20.0)]
+    r = [e[0] / e[1], e[1] / e[2]]
+    print(f"[rk order] dof={dof} RK{scheme}: local errors {e}, ratios {r}", flush=True)
+    assert all(0.85 * want <= x <= 1.15 * want for x in r), (e, r)
+    ep = [_interior_local_error(planted_step, dof, n, st, dt) for dt in (40.0, 20.0)]
+    assert ep[0] / ep[1] < 0.6 * want, ep
 
 
 def test_exact_division_helper_is_ieee_division(tmp_path):
