@@ -50,9 +50,10 @@ size_t footprint_bytes(const mcv_config *c) {
   b += (fp32c ? 3 : 9) * d(np);                       // FP64 levels
   if (!fp32c) b += 3 * d(np);                          // RK4 accumulator
   b += d(nband) + 2 * d(np) + d(4 * diag_blocks) + d(2) + 6 * d(np);
-  if (fp32c)
-    b += 3 * d(np) + d(np) + 3 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) + 2 * f(nband) +
-         f(np);  // join, phi_ref, FP32 statics, levels, compensation, accumulator, bands
+  if (fp32c)  // join, phi_ref, phi_ref*sqrtG, FP32 Coriolis and reference gradient pair,
+              // levels, compensation, accumulator, bands
+    b += 3 * d(np) + d(np) + d(np) + 3 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) +
+         2 * f(nband);
   return b;
 }
 
