@@ -167,9 +167,9 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       sU[li] = u;
       sV[li] = v;
       sPG[li] = pg;
-      sCU[li] = cu;
-      sCV[li] = cv;
       const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
+      if (iny) sCU[TL::cu_at(lx, ly)] = cu;
+      if (inx) sCV[TL::cv_at(lx, ly)] = cv;
       if (inx && iny) {
         const int ox = lx - H, oy = ly - H;
         const C phi = (C)((double)pg * rrs[q]);
@@ -197,28 +197,48 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
     const int st = isx ? 1 : TX;
     const int b = isx ? (o + H) * TX + (H + mb * H) : (H + mb * H) * TX + (o + H);
     const C *sC = isx ? sCU : sCV;
+    const int cst = isx ? 1 : NX;  // the same line in the contravariant wind's slots
+    const int cb = isx ? TL::cu_at(H + mb * H, o + H) : TL::cv_at(o + H, H + mb * H);
     const C *sQ = isx ? sU : sV;
     const C *sW = isx ? sV : sU;
-    C eL[DOF], eR[DOF], qL[DOF], qR[DOF], dL[DOF], dR[DOF], pL[DOF], pR[DOF], wL[DOF], wR[DOF];
-#pragma unroll
-    for (int k = 0; k < DOF; ++k) {
-      const int l = b - H * st + k * st, r = b + k * st;
-      eL[k] = sE[l];  eR[k] = sE[r];
-      qL[k] = sQ[l];  qR[k] = sQ[r];
-      pL[k] = sPG[l]; pR[k] = sPG[r];
-      dL[k] = sPG[l] * sC[l];
-      dR[k] = sPG[r] * sC[r];
-      wL[k] = sW[l];  wR[k] = sW[r];
-    }
     const C lam = isx ? lamx[tt] : lamy[tt];
-    const C c = sC[b];
+    const C c = sC[cb];
     const C up = c == C(0) ? C(1) : copysign(C(1), c);
-    const C fF = riemann_t(fit_right_t(eL, DOF, rdx, r2dx), fit_left_t(eR, DOF, rdx, r2dx),
-                           fit_right_t(qL, DOF, rdx, r2dx), fit_left_t(qR, DOF, rdx, r2dx), lam);
-    const C fD = riemann_t(fit_right_t(dL, DOF, rdx, r2dx), fit_left_t(dR, DOF, rdx, r2dx),
-                           fit_right_t(pL, DOF, rdx, r2dx), fit_left_t(pR, DOF, rdx, r2dx), lam);
-    const C wlr = fit_right_t(wL, DOF, rdx, r2dx), wrl = fit_left_t(wR, DOF, rdx, r2dx);
-    const C fW = riemann_t(wlr, wrl, wlr, wrl, up);
+    // the three flux pairs one after the other (only one pair's windows live)
+    C fF, fD, fW;
+    {
+      C eL[DOF], eR[DOF], qL[DOF], qR[DOF];
+#pragma unroll
+      for (int k = 0; k < DOF; ++k) {
+        const int l = b - H * st + k * st, r = b + k * st;
+        eL[k] = sE[l];  eR[k] = sE[r];
+        qL[k] = sQ[l];  qR[k] = sQ[r];
+      }
+      fF = riemann_t(fit_right_t(eL, DOF, rdx, r2dx), fit_left_t(eR, DOF, rdx, r2dx),
+                     fit_right_t(qL, DOF, rdx, r2dx), fit_left_t(qR, DOF, rdx, r2dx), lam);
+    }
+    {
+      C dL[DOF], dR[DOF], pL[DOF], pR[DOF];
+#pragma unroll
+      for (int k = 0; k < DOF; ++k) {
+        const int l = b - H * st + k * st, r = b + k * st;
+        pL[k] = sPG[l]; pR[k] = sPG[r];
+        dL[k] = sPG[l] * sC[cb - H * cst + k * cst];
+        dR[k] = sPG[r] * sC[cb + k * cst];
+      }
+      fD = riemann_t(fit_right_t(dL, DOF, rdx, r2dx), fit_left_t(dR, DOF, rdx, r2dx),
+                     fit_right_t(pL, DOF, rdx, r2dx), fit_left_t(pR, DOF, rdx, r2dx), lam);
+    }
+    {
+      C wL[DOF], wR[DOF];
+#pragma unroll
+      for (int k = 0; k < DOF; ++k) {
+        wL[k] = sW[b - H * st + k * st];
+        wR[k] = sW[b + k * st];
+      }
+      const C wlr = fit_right_t(wL, DOF, rdx, r2dx), wrl = fit_left_t(wR, DOF, rdx, r2dx);
+      fW = riemann_t(wlr, wrl, wlr, wrl, up);
+    }
     if (isx) {
       rxF[tt] = fF; rxD[tt] = fD; rxV[tt] = fW;
     } else {
@@ -312,7 +332,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           C fe[DOF], fd[DOF], fv[DOF];
 #pragma unroll
           for (int k = 0; k < DOF; ++k) {
-            fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
+            fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k - H * TX]; fv[k] = sV[b0 + k];
           }
           const int r1 = r0 + NY;
           fx = cell_interior_t<DOF>(fe, rxF[r0], rxF[r1], pos, dxf, rdx, rdxx);
@@ -331,7 +351,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
 #pragma unroll
           for (int k = 0; k < DOF; ++k) {
             const int l = b0 + k * TX;
-            fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
+            fe[k] = sE[l]; fd[k] = sPG[l] * sCV[(H + mb * H + k) * NX + ox]; fu[k] = sU[l];
           }
           const int r1 = r0 + NX;
           fy = cell_interior_t<DOF>(fe, ryF[r0], ryF[r1], pos, dxf, rdx, rdxx);
@@ -344,7 +364,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       // tendencies assembled in FP64 (spatial_operators_mod.F90:127-131)
       const double sg = rsg[q], rsgv = rrs[q];
       const double fc = (double)rf[q];
-      const double cu = sCU[li], cv = sCV[li];
+      const double cu = sCU[TL::cu_at(lx, ly)], cv = sCV[TL::cv_at(lx, ly)];
       const double vort = ((double)vx - (double)uy) * rsgv;
       const double kpg = -(double)dvx - (double)dvy;
       const double ku = -((double)rdx_[q] + (double)fx) + sg * (vort + fc) * cv;

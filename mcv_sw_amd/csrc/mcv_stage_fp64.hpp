@@ -29,16 +29,24 @@ struct TileLayout {
   static constexpr int kPts = TX * TY;
   static constexpr int kEx = (CX + 1) * NY;  // x-sweep cell ends: (CX+1) per core row
   static constexpr int kEy = (CY + 1) * NX;  // y-sweep: (CY+1) per core column
-  static constexpr size_t kBytes = sizeof(T) * (6 * kPts + 4 * kEx + 4 * kEy);
+  // The contravariant winds are read only along their own sweep: contraU on
+  // the core rows (NY x TX), contraV on the core columns (TY x NX).  The
+  // cell-end wave speeds share the storage of the first flux of their sweep:
+  // each phase-1 task reads its wave speed before it writes that flux.
+  static constexpr int kCU = NY * TX, kCV = TY * NX;
+  static constexpr size_t kBytes = sizeof(T) * (4 * kPts + kCU + kCV + 3 * kEx + 3 * kEy);
   T *sE, *sU, *sV, *sPG, *sCU, *sCV, *lamx, *lamy, *rxF, *rxD, *rxV, *ryF, *ryD, *ryU;
   __device__ __forceinline__ explicit TileLayout(char *smem) {
     T *q = reinterpret_cast<T *>(smem);
-    sE = q; q += kPts; sU = q; q += kPts; sV = q; q += kPts;
-    sPG = q; q += kPts; sCU = q; q += kPts; sCV = q; q += kPts;
-    lamx = q; q += kEx; lamy = q; q += kEy;
+    sE = q; q += kPts; sU = q; q += kPts; sV = q; q += kPts; sPG = q; q += kPts;
+    sCU = q; q += kCU; sCV = q; q += kCV;
     rxF = q; q += kEx; rxD = q; q += kEx; rxV = q; q += kEx;
     ryF = q; q += kEy; ryD = q; q += kEy; ryU = q;
+    lamx = rxF; lamy = ryF;
   }
+  // contraU / contraV slots of staged point (lx, ly): a core row / column
+  static __device__ __forceinline__ int cu_at(int lx, int ly) { return (ly - H) * TX + lx; }
+  static __device__ __forceinline__ int cv_at(int lx, int ly) { return ly * NX + (lx - H); }
 };
 
 // One (panel p, tile tx, ty) of the spatial operator + RK stage `kind`.  The
@@ -128,9 +136,9 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       sU[li] = u;
       sV[li] = v;
       sPG[li] = phi * rsg[q];
-      sCU[li] = cu;
-      sCV[li] = cv;
       const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
+      if (iny) sCU[TL::cu_at(lx, ly)] = cu;
+      if (inx) sCV[TL::cv_at(lx, ly)] = cv;
       if (inx && iny) {
         const int ox = lx - H, oy = ly - H;
         if (ox % H == 0) lamx[(ox / H) * NY + oy] = wave_speed(cu, phi, g11);
@@ -152,28 +160,48 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
     const int st = isx ? 1 : TX;  // stride along the sweep
     const int b = isx ? (o + H) * TX + (H + mb * H) : (H + mb * H) * TX + (o + H);
     const double *sC = isx ? sCU : sCV;
+    const int cst = isx ? 1 : NX;  // the same line in the contravariant wind's slots
+    const int cb = isx ? TL::cu_at(H + mb * H, o + H) : TL::cv_at(o + H, H + mb * H);
     const double *sQ = isx ? sU : sV;   // q of the gradient flux
     const double *sW = isx ? sV : sU;   // upwinded cross wind
-    double eL[DOF], eR[DOF], qL[DOF], qR[DOF], dL[DOF], dR[DOF], pL[DOF], pR[DOF], wL[DOF],
-        wR[DOF];
-#pragma unroll
-    for (int k = 0; k < DOF; ++k) {
-      const int l = b - H * st + k * st, r = b + k * st;
-      eL[k] = sE[l];  eR[k] = sE[r];
-      qL[k] = sQ[l];  qR[k] = sQ[r];
-      pL[k] = sPG[l]; pR[k] = sPG[r];
-      dL[k] = sPG[l] * sC[l];
-      dR[k] = sPG[r] * sC[r];
-      wL[k] = sW[l];  wR[k] = sW[r];
-    }
     const double lam = isx ? lamx[tt] : lamy[tt];
-    const double up = upwind_sign(sC[b]);
-    const double fF = riemann(fit_right<DOF>(eL, dv), fit_left<DOF>(eR, dv),
-                              fit_right<DOF>(qL, dv), fit_left<DOF>(qR, dv), lam);
-    const double fD = riemann(fit_right<DOF>(dL, dv), fit_left<DOF>(dR, dv),
-                              fit_right<DOF>(pL, dv), fit_left<DOF>(pR, dv), lam);
-    const double wlr = fit_right<DOF>(wL, dv), wrl = fit_left<DOF>(wR, dv);
-    const double fW = riemann(wlr, wrl, wlr, wrl, up);
+    const double up = upwind_sign(sC[cb]);
+    // the three flux pairs one after the other, each from its own LDS reads,
+    // so that only one pair's windows are live at a time
+    double fF, fD, fW;
+    {
+      double eL[DOF], eR[DOF], qL[DOF], qR[DOF];
+#pragma unroll
+      for (int k = 0; k < DOF; ++k) {
+        const int l = b - H * st + k * st, r = b + k * st;
+        eL[k] = sE[l];  eR[k] = sE[r];
+        qL[k] = sQ[l];  qR[k] = sQ[r];
+      }
+      fF = riemann(fit_right<DOF>(eL, dv), fit_left<DOF>(eR, dv), fit_right<DOF>(qL, dv),
+                   fit_left<DOF>(qR, dv), lam);
+    }
+    {
+      double dL[DOF], dR[DOF], pL[DOF], pR[DOF];
+#pragma unroll
+      for (int k = 0; k < DOF; ++k) {
+        const int l = b - H * st + k * st, r = b + k * st;
+        pL[k] = sPG[l]; pR[k] = sPG[r];
+        dL[k] = sPG[l] * sC[cb - H * cst + k * cst];
+        dR[k] = sPG[r] * sC[cb + k * cst];
+      }
+      fD = riemann(fit_right<DOF>(dL, dv), fit_left<DOF>(dR, dv), fit_right<DOF>(pL, dv),
+                   fit_left<DOF>(pR, dv), lam);
+    }
+    {
+      double wL[DOF], wR[DOF];
+#pragma unroll
+      for (int k = 0; k < DOF; ++k) {
+        wL[k] = sW[b - H * st + k * st];
+        wR[k] = sW[b + k * st];
+      }
+      const double wlr = fit_right<DOF>(wL, dv), wrl = fit_left<DOF>(wR, dv);
+      fW = riemann(wlr, wrl, wlr, wrl, up);
+    }
     if (isx) {
       rxF[tt] = fF; rxD[tt] = fD; rxV[tt] = fW;
     } else {
@@ -257,7 +285,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
           double fe[DOF], fd[DOF], fv[DOF];
 #pragma unroll
           for (int k = 0; k < DOF; ++k) {
-            fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k]; fv[k] = sV[b0 + k];
+            fe[k] = sE[b0 + k]; fd[k] = sPG[b0 + k] * sCU[b0 + k - H * TX]; fv[k] = sV[b0 + k];
           }
           const int r1 = r0 + NY;
           fx = cell_interior<DOF>(fe, rxF[r0], rxF[r1], pos, dx, dv);
@@ -276,7 +304,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
 #pragma unroll
           for (int k = 0; k < DOF; ++k) {
             const int l = b0 + k * TX;
-            fe[k] = sE[l]; fd[k] = sPG[l] * sCV[l]; fu[k] = sU[l];
+            fe[k] = sE[l]; fd[k] = sPG[l] * sCV[(H + mb * H + k) * NX + ox]; fu[k] = sU[l];
           }
           const int r1 = r0 + NX;
           fy = cell_interior<DOF>(fe, ryF[r0], ryF[r1], pos, dx, dv);
@@ -289,7 +317,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       const Recip sgr{rsg[q], rrs[q]};
       const double sg = sgr.b;
       const double fc = rf[q];
-      const double cu = sCU[li], cv = sCV[li];
+      const double cu = sCU[TL::cu_at(lx, ly)], cv = sCV[TL::cv_at(lx, ly)];
       // spatial_operators_mod.F90:127-131
       const double vort = div_by(vx - uy, sgr);
       const double kpg = -dvx - dvy;
@@ -356,7 +384,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
 // Single-solve stage kernel: one workgroup per (panel, tile), blocks of one
 // XCD on neighbouring tiles.
 template <int DOF, int KIND, int CX, int CY>
-__global__ __launch_bounds__(kStageThreads, 5) void stage_kernel(MeshDev m, StageArgs a,
+__global__ __launch_bounds__(kStageThreads, 6) void stage_kernel(MeshDev m, StageArgs a,
                                                               int tiles_x) {
   __shared__ __align__(16) char smem[TileLayout<double, DOF, CX, CY>::kBytes];
   const int L = xcd_remap(blockIdx.x, gridDim.x);
