@@ -504,6 +504,7 @@ def main():
     bytes_per_launch = bpcs * cells / 4
     avg_launch_s = rec64["stage_kernel_ms_per_step"] / 4 / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = measured_traffic(os.path.join(ROOT, "profiles", "stage_kernel_mcv4_n384_fp64.md"))
     res = {
         "metric": "SW cell-steps/s (plain FP64 vs batched FP32) at stated h err; serve p50/p99",
         "value": rec64["value"],
@@ -524,8 +525,9 @@ def main():
                    "h_rel_err_vs_fp64": 0.0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": measured_traffic(os.path.join(
-                         ROOT, "profiles", "stage_kernel_mcv4_n384_fp64.md")),
+                     "traffic": traffic,
+                     # the measured HBM bytes per launch over this run's mean launch time
+                     "traffic_GBs": traffic / avg_launch_s / 1e9 if traffic else None,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "kernel": "stage_kernel (4 launches per RK4 step)",
                      "bytes_per_cell_step": bpcs,
@@ -575,6 +577,9 @@ def main():
         res["roofline"][f"stream_copy_{gib}GiB_GBs"] = gbs
         best = max(best, gbs)
     res["roofline"]["frac_of_stream_copy"] = achieved / best
+    if res["roofline"].get("traffic_GBs"):
+        res["roofline"]["traffic_frac_of_stream_copy_4GiB"] = (
+            res["roofline"]["traffic_GBs"] / res["roofline"]["stream_copy_4GiB_GBs"])
     if not args.no_cpu_baseline:
         try:
             res["cpu_baseline"] = cpu_baseline(lambda: hostlib.mesh(DOF, N))
