@@ -127,7 +127,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   for (int t0 = threadIdx.x; t0 < TX * TY; t0 += kP0 * blockDim.x) {
     float rdp[kP0], ru[kP0], rv[kP0];
     double rpg[kP0];
-    double r11[kP0], r12[kP0], r22[kP0], rsg[kP0], rrs[kP0];
+    double r11[kP0], r12[kP0], r22[kP0], rsg[kP0];
     bool live[kP0];
 #pragma unroll
     for (int q = 0; q < kP0; ++q) {
@@ -143,12 +143,11 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       ru[q] = pu.b;
       rv[q] = ldg(a.in_v, g);
       rpg[q] = ldg(m.pgref, g);
-      const D2 ga = ldg(mpk, 2 * off), gb = ldg(mpk, 2 * off + 1), sr = ldg(spk, off);
+      const D2 ga = ldg(mpk, 2 * off), gb = ldg(mpk, 2 * off + 1);
+      rsg[q] = ga.a;
       r11[q] = ga.b;
       r12[q] = gb.a;
       r22[q] = gb.b;
-      rsg[q] = sr.a;
-      rrs[q] = sr.b;
     }
 #pragma unroll
     for (int q = 0; q < kP0; ++q) {
@@ -172,7 +171,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       if (inx) sCV[TL::cv_at(lx, ly)] = cv;
       if (inx && iny) {
         const int ox = lx - H, oy = ly - H;
-        const C phi = (C)((double)pg * rrs[q]);
+        const C phi = (C)((double)pg / rsg[q]);  // for the wave speeds only
         if (ox % H == 0) {
           const C sq = sqrt((C)g11 * phi);
           lamx[(ox / H) * NY + oy] = fmax(fabs(cu - sq), fabs(cu + sq));
