@@ -120,7 +120,6 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   const uint32_t pbase = (uint32_t)(p - 1) * (uint32_t)(nh * nh);
   const uint32_t mbase = (uint32_t)(p - 1) * (uint32_t)m.metric_pstride;
   const D2 *const mpk = m.mpack + 2 * (size_t)mbase;  // this panel's packed metric terms
-  const D2 *const spk = m.spack + (size_t)mbase;
 
   // loads of all a thread's points first, then the derived values
   constexpr int kP0 = (TX * TY + kStageThreads - 1) / kStageThreads;
@@ -171,6 +170,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       if (inx) sCV[TL::cv_at(lx, ly)] = cv;
       if (inx && iny) {
         const int ox = lx - H, oy = ly - H;
+        lds.sSG[TL::core_at(lx, ly)] = rsg[q];
         const C phi = (C)((double)pg / rsg[q]);  // for the wave speeds only
         if (ox % H == 0) {
           const C sq = sqrt((C)g11 * phi);
@@ -258,7 +258,6 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   for (int t0 = threadIdx.x; t0 < kSlots + kTail; t0 += kP2 * blockDim.x) {
     int rox[kP2], roy[kP2];
     bool live[kP2];
-    double rsg[kP2], rrs[kP2];
     float rf[kP2], rdx_[kP2], rdy_[kP2], rin[kP2], rbp[kP2], rbu[kP2], rbv[kP2], rcp[kP2],
         rcu[kP2], rcv[kP2], rap[kP2], rau[kP2], rav[kP2];
 #pragma unroll
@@ -287,9 +286,6 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       if (!ok) continue;
       const uint32_t off = (uint32_t)((j0 + oy + H - 1) * nh + (i0 + ox + H - 1));
       const uint32_t g = pbase + off;
-      const D2 sr = ldg(spk, off);
-      rsg[q] = sr.a;
-      rrs[q] = sr.b;
       rf[q] = ldg(m.f, g);
       const F2 de = ldg(m.dE, g);
       rdx_[q] = de.a;
@@ -361,7 +357,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       const int i = i0 + ox, j = j0 + oy;
       const uint32_t g = pbase + (uint32_t)((j + H - 1) * nh + (i + H - 1));
       // tendencies assembled in FP64 (spatial_operators_mod.F90:127-131)
-      const double sg = rsg[q], rsgv = rrs[q];
+      const double sg = lds.sSG[TL::core_at(lx, ly)], rsgv = 1.0 / sg;
       const double fc = (double)rf[q];
       const double cu = sCU[TL::cu_at(lx, ly)], cv = sCV[TL::cv_at(lx, ly)];
       const double vort = ((double)vx - (double)uy) * rsgv;

@@ -34,10 +34,15 @@ struct TileLayout {
   // cell-end wave speeds share the storage of the first flux of their sweep:
   // each phase-1 task reads its wave speed before it writes that flux.
   static constexpr int kCU = NY * TX, kCV = TY * NX;
-  static constexpr size_t kBytes = sizeof(T) * (4 * kPts + kCU + kCV + 3 * kEx + 3 * kEy);
+  // sqrtG at the core points (FP64 whatever T is), staged for phase 2
+  static constexpr int kCore = NX * NY;
+  static constexpr size_t kBytes =
+      sizeof(double) * kCore + sizeof(T) * (4 * kPts + kCU + kCV + 3 * kEx + 3 * kEy);
+  double *sSG;
   T *sE, *sU, *sV, *sPG, *sCU, *sCV, *lamx, *lamy, *rxF, *rxD, *rxV, *ryF, *ryD, *ryU;
   __device__ __forceinline__ explicit TileLayout(char *smem) {
-    T *q = reinterpret_cast<T *>(smem);
+    sSG = reinterpret_cast<double *>(smem);
+    T *q = reinterpret_cast<T *>(smem + sizeof(double) * kCore);
     sE = q; q += kPts; sU = q; q += kPts; sV = q; q += kPts; sPG = q; q += kPts;
     sCU = q; q += kCU; sCV = q; q += kCV;
     rxF = q; q += kEx; rxD = q; q += kEx; rxV = q; q += kEx;
@@ -47,6 +52,9 @@ struct TileLayout {
   // contraU / contraV slots of staged point (lx, ly): a core row / column
   static __device__ __forceinline__ int cu_at(int lx, int ly) { return (ly - H) * TX + lx; }
   static __device__ __forceinline__ int cv_at(int lx, int ly) { return ly * NX + (lx - H); }
+  static __device__ __forceinline__ int core_at(int lx, int ly) {
+    return (ly - H) * NX + (lx - H);
+  }
 };
 
 // One (panel p, tile tx, ty) of the spatial operator + RK stage `kind`.  The
@@ -81,7 +89,6 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
   const uint32_t mbase = (uint32_t)(p - 1) * (uint32_t)m.metric_pstride;
   const uint32_t sbase = (uint32_t)(p - 1) * (uint32_t)m.phis_pstride;
   const D2 *const mpk = m.mpack + 2 * (size_t)mbase;  // this panel's packed metric terms
-  const D2 *const spk = m.spack + (size_t)mbase;
 
   // Phase 0: stage the cross-shaped tile + halo; derived fields per point.
   // Every thread first issues the loads of all its points, then derives and
@@ -141,6 +148,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       if (inx) sCV[TL::cv_at(lx, ly)] = cv;
       if (inx && iny) {
         const int ox = lx - H, oy = ly - H;
+        lds.sSG[TL::core_at(lx, ly)] = rsg[q];
         if (ox % H == 0) lamx[(ox / H) * NY + oy] = wave_speed(cu, phi, g11);
         if (oy % H == 0) lamy[(oy / H) * NX + ox] = wave_speed(cv, phi, g22);
       }
@@ -224,7 +232,7 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
   for (int t0 = threadIdx.x; t0 < kSlots + kTail; t0 += kP2 * blockDim.x) {
     int rox[kP2], roy[kP2];
     bool live[kP2];
-    double rsg[kP2], rrs[kP2], rf[kP2], rbp[kP2], rbu[kP2], rbv[kP2], rap[kP2], rau[kP2],
+    double rf[kP2], rbp[kP2], rbu[kP2], rbv[kP2], rap[kP2], rau[kP2],
         rav[kP2];
 #pragma unroll
     for (int q = 0; q < kP2; ++q) {
@@ -252,9 +260,6 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       if (!ok) continue;
       const uint32_t off = (uint32_t)((j0 + oy + H - 1) * nh + (i0 + ox + H - 1));
       const uint32_t g = pbase + off;
-      const D2 sr = ldg(spk, off);
-      rsg[q] = sr.a;
-      rrs[q] = sr.b;
       rf[q] = ldg(m.f, g);
       if (need_base) {
         const D2 bpu = ldg(a.b_pu, g);
@@ -314,8 +319,8 @@ __device__ __forceinline__ void stage_tile(const MeshDev &m, const StageArgs &a,
       }
       const int i = i0 + ox, j = j0 + oy;
       const uint32_t g = pbase + (uint32_t)((j + H - 1) * nh + (i + H - 1));
-      const Recip sgr{rsg[q], rrs[q]};
-      const double sg = sgr.b;
+      const double sg = lds.sSG[TL::core_at(lx, ly)];
+      const Recip sgr{sg, 1.0 / sg};  // the IEEE quotient, as the host forms RN(1/sqrtG)
       const double fc = rf[q];
       const double cu = sCU[TL::cu_at(lx, ly)], cv = sCV[TL::cv_at(lx, ly)];
       // spatial_operators_mod.F90:127-131

@@ -6,8 +6,9 @@ product's flags).  Compiles on the CPU; no GPU needed.
   - the 8-wave builds of the batched stage kernel reach 8 waves per SIMD
     (their extra SGPRs spill to VGPR lanes only);
   - the FP64 MCV4 single-solve stage kernels at the 6x4 and 5x5 cell tiles
-    (the shapes the autotuner takes at 6x384^2, whose LDS, 25-26 KB, allows
-    6 workgroups per CU) stay within 80 VGPRs and run 6 waves per SIMD.
+    (the shapes the autotuner takes at 6x384^2, whose LDS, 27-28 KB, allows
+    5 workgroups per CU) stay within 80 VGPRs and run at least 5 waves per
+    SIMD.
 """
 import os
 import re
@@ -69,4 +70,4 @@ def test_fp64_stage_kernels_within_80_vgprs(kernels):
     fp64 = {k: v for k, v in kernels.items()
             if re.match(r"_ZN3mcv12stage_kernelILi4ELi\dELi(6ELi4|5ELi5)E", k)}
     assert len(fp64) == 8 * 2  # stage kinds (tendency + 7) x tile shapes
-    assert all(v["vgpr"] <= 80 and v["occupancy"] == 6 for v in fp64.values()), fp64
+    assert all(v["vgpr"] <= 80 and v["occupancy"] >= 5 for v in fp64.values()), fp64
