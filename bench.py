@@ -70,13 +70,13 @@ def fp32c_bytes_per_point_step(scheme):
     """Algorithmic HBM bytes per unique point per time step of the FP32C stage
     kernels: FP32 stage input, base hi (stages 2..), base lo (last stage),
     RK4 accumulator, output and (last stage) output residue, plus per stage
-    the FP64 phi_ref*sqrtG, the FP32 reference gradient (2 words) and the FP32
-    Coriolis parameter."""
+    the FP64 phi_ref*sqrtG and one 16-byte FP32 record of the reference
+    gradient (2 words) and the Coriolis parameter."""
     if scheme == "RK4":
-        # in 4x12, hi 3x12, lo 12, acc 12+24+24+12, out 4x12 + lo 12, static 4 x 20
-        return 48 + 36 + 12 + 72 + 48 + 12 + 4 * 20
-    # RK3: in 3x12, hi 2x12, lo 12, out 3x12 + lo 12, static 3 x 20
-    return 36 + 24 + 12 + 36 + 12 + 3 * 20
+        # in 4x12, hi 3x12, lo 12, acc 12+24+24+12, out 4x12 + lo 12, static 4 x 24
+        return 48 + 36 + 12 + 72 + 48 + 12 + 4 * 24
+    # RK3: in 3x12, hi 2x12, lo 12, out 3x12 + lo 12, static 3 x 24
+    return 36 + 24 + 12 + 36 + 12 + 3 * 24
 
 
 def measured_traffic(path):

@@ -58,6 +58,10 @@ struct alignas(16) D2 {
 struct alignas(8) F2 {
   float a, b;
 };
+// Four floats loaded together (one 16-byte load).
+struct alignas(16) F4 {
+  float a, b, c, d;
+};
 
 struct MeshDev {
   int dof, n, npv, nh;

@@ -34,9 +34,10 @@ struct MeshF32 {
   long long metric_pstride;
   const double *sg, *rsg, *ig11, *ig12, *ig22;  // FP64 metric (same arrays as the FP64 path)
   const D2 *mpack, *spack;      // the same terms packed per point (MeshDev::mpack, spack)
-  const float *f;               // RN32(Coriolis parameter)
   const double *pgref;          // phi_ref * sqrtG (FP64: a rounded copy is a fixed mass-flux error)
-  const F2 *dE;                 // RN32 of the FP64 MCV derivatives (x, y) of phi_ref + phi_s
+  // {dE/dx, dE/dy, f, 0}: RN32 of the FP64 MCV derivatives (x, y) of
+  // phi_ref + phi_s and of the Coriolis parameter, one 16-byte load
+  const F4 *fdE;
 };
 
 struct StageArgs32 {
@@ -286,10 +287,10 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       if (!ok) continue;
       const uint32_t off = (uint32_t)((j0 + oy + H - 1) * nh + (i0 + ox + H - 1));
       const uint32_t g = pbase + off;
-      rf[q] = ldg(m.f, g);
-      const F2 de = ldg(m.dE, g);
-      rdx_[q] = de.a;
-      rdy_[q] = de.b;
+      const F4 fde = ldg(m.fdE, g);
+      rdx_[q] = fde.a;
+      rdy_[q] = fde.b;
+      rf[q] = fde.c;
       if (kind == kTend) continue;
       if (last) {
         const F2 cpu = ldg(a.c_pu, g);
@@ -578,13 +579,6 @@ __global__ __launch_bounds__(256) void boundary_kernel_f32(MeshDev m, BoundaryAr
     boundary_item_f32<DOF>(m, a, t);
 }
 
-// RN32 copy of an FP64 array.
-static __global__ void round_to_f32_kernel(long long n, const double *x, float *y) {
-  for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < n;
-       g += (long long)gridDim.x * blockDim.x)
-    y[g] = (float)x[g];
-}
-
 // pgref = phi_ref * sqrtG on every point.
 static __global__ void pgref_kernel(MeshDev m, const double *phi_ref, double *pgref) {
   const long long per = (long long)m.nh * m.nh;
@@ -615,9 +609,10 @@ static __global__ void band_copy_kernel(MeshDev m, const D2 *pu, const double *v
 
 // FP64 MCV derivatives (x and y) of E_ref = phi_ref + phi_s with the
 // dissipation term left out: the part of dE/dx, dE/dy that does not depend on
-// the evolving state.  One thread per owned point; runs once per solve.
+// the evolving state, rounded to FP32 beside the rounded Coriolis parameter.
+// One thread per owned point; runs once per solve.
 template <int DOF>
-__global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, F2 *dE) {
+__global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, F4 *fdE) {
   constexpr int H = DOF - 1;
   const int npv = m.npv, nh = m.nh;
   const long long per = (long long)nh * nh;
@@ -648,7 +643,7 @@ __global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, F2 *dE) {
       return cell_interior<DOF>(f, dL, central(bL + H), pos, m.dx, m.dv);
     };
     const long long g = pidx(nh, H, p, i, j);
-    dE[g] = F2{(float)deriv(true), (float)deriv(false)};
+    fdE[g] = F4{(float)deriv(true), (float)deriv(false), (float)m.f[g], 0.0f};
   }
 }
 

@@ -50,9 +50,9 @@ size_t footprint_bytes(const mcv_config *c) {
   b += (fp32c ? 3 : 9) * d(np);                       // FP64 levels
   if (!fp32c) b += 3 * d(np);                          // RK4 accumulator
   b += d(nband) + 2 * d(np) + d(4 * diag_blocks) + d(2) + 6 * d(np);
-  if (fp32c)  // join, phi_ref, phi_ref*sqrtG, FP32 Coriolis and reference gradient pair,
+  if (fp32c)  // join, phi_ref, phi_ref*sqrtG, FP32 {reference gradient, Coriolis, 0},
               // levels, compensation, accumulator, bands
-    b += 3 * d(np) + d(np) + d(np) + 3 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) +
+    b += 3 * d(np) + d(np) + d(np) + 4 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) +
          2 * f(nband);
   return b;
 }
@@ -431,20 +431,14 @@ int setup32(mcv_handle *h) {
                      h->phi_ref, nullptr);
   HIP_TRY(hipGetLastError());
   h->first_pending = false;  // the FP32C stages form the contravariant winds themselves
-  F2 *dE;
-  float *f32;
-  if ((rc = dev_alloc_t(h, &dE, np)) ||
-      (rc = dev_alloc_t(h, &f32, np)))
-    return rc;
-  hipLaunchKernelGGL(round_to_f32_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->mesh.f,
-                     f32);
-  HIP_TRY(hipGetLastError());
+  F4 *fdE;
+  if ((rc = dev_alloc_t(h, &fdE, np))) return rc;
   if (h->cfg.dof == 3)
     hipLaunchKernelGGL(ref_gradient_kernel<3>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
-                       h->phi_ref, dE);
+                       h->phi_ref, fdE);
   else
     hipLaunchKernelGGL(ref_gradient_kernel<4>, dim3(2048), dim3(256), 0, h->stream, h->mesh,
-                       h->phi_ref, dE);
+                       h->phi_ref, fdE);
   HIP_TRY(hipGetLastError());
   for (int l = 0; l < 3; ++l)
     if ((rc = dev_alloc_t(h, &h->l32pu[l], np)) || (rc = dev_alloc_t(h, &h->l32v[l], np)))
@@ -487,9 +481,8 @@ int setup32(mcv_handle *h) {
   m.ig22 = d.ig22;
   m.mpack = d.mpack;
   m.spack = d.spack;
-  m.f = f32;
   m.pgref = pgref;
-  m.dE = dE;
+  m.fdE = fdE;
   HIP_TRY(hipStreamSynchronize(h->stream));
   return MCV_OK;
 }
