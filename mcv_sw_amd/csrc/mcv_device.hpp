@@ -58,6 +58,11 @@ struct alignas(16) D2 {
 struct alignas(8) F2 {
   float a, b;
 };
+// Three floats as one 12-byte record (one dwordx3 load or store; 4-byte
+// alignment is all the instruction needs): an FP32C point's {phi', u, v}.
+struct alignas(4) F3 {
+  float a, b, c;
+};
 // Four floats loaded together (one 16-byte load).
 struct alignas(16) F4 {
   float a, b, c, d;

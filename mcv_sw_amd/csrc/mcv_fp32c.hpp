@@ -41,13 +41,13 @@ struct MeshF32 {
 };
 
 struct StageArgs32 {
-  // each level, compensation and accumulator as {phi', u} pairs and v
-  const F2 *in_pu; const float *in_v;      // stage state (phi', u, v)
-  const F2 *b_pu; const float *b_v;        // base Y_n (hi)
-  const F2 *c_pu; const float *c_v;        // base Y_n compensation (lo)
-  F2 *acc_pu; float *acc_v;                // RK4 accumulator
-  F2 *o_pu; float *o_v;                    // stage output (raw interior)
-  F2 *oc_pu; float *oc_v;                  // compensation of the output (last stage only)
+  // each level, compensation and accumulator as one {phi', u, v} record per point
+  const F3 *in;                            // stage state
+  const F3 *b;                             // base Y_n (hi)
+  const F3 *c;                             // base Y_n compensation (lo)
+  F3 *acc;                                 // RK4 accumulator
+  F3 *o;                                   // stage output (raw interior)
+  F3 *oc;                                  // compensation of the output (last stage only)
   float *band_raw;                         // raw band lines (hi), see band_slot()
   float *band_raw_c;                       // raw band lines (lo), last stage only
   double c1;                               // stage increment coefficient (x dt)
@@ -125,7 +125,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   // loads of all a thread's points first, then the derived values
   constexpr int kP0 = (TX * TY + kStageThreads - 1) / kStageThreads;
   for (int t0 = threadIdx.x; t0 < TX * TY; t0 += kP0 * blockDim.x) {
-    float rdp[kP0], ru[kP0], rv[kP0];
+    F3 rs[kP0];
     double rpg[kP0];
     double r11[kP0], r12[kP0], r22[kP0], rsg[kP0];
     bool live[kP0];
@@ -138,10 +138,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       if (!live[q]) continue;
       const uint32_t off = (uint32_t)((j0 - 1 + ly) * nh + (i0 - 1 + lx));
       const uint32_t g = pbase + off;
-      const F2 pu = ldg(a.in_pu, g);
-      rdp[q] = pu.a;
-      ru[q] = pu.b;
-      rv[q] = ldg(a.in_v, g);
+      rs[q] = ldg(a.in, g);
       rpg[q] = ldg(m.pgref, g);
       const D2 ga = ldg(mpk, 2 * off), gb = ldg(mpk, 2 * off + 1);
       rsg[q] = ga.a;
@@ -154,7 +151,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       if (!live[q]) continue;
       const int li = t0 + q * blockDim.x;  // = ly * TX + lx
       const int ly = li / TX, lx = li - ly * TX;
-      const float dp = rdp[q], u = ru[q], v = rv[q];
+      const float dp = rs[q].a, u = rs[q].b, v = rs[q].c;
       // metric products in FP64 (the FP32 inputs are exact in FP64), rounded once
       const double ud = u, vd = v;
       const double g11 = r11[q], g12 = r12[q], g22 = r22[q];
@@ -293,23 +290,24 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       rf[q] = fde.c;
       if (kind == kTend) continue;
       if (last) {
-        const F2 cpu = ldg(a.c_pu, g);
-        rcp[q] = cpu.a;
-        rcu[q] = cpu.b;
-        rcv[q] = ldg(a.c_v, g);
+        const F3 cc = ldg(a.c, g);
+        rcp[q] = cc.a;
+        rcu[q] = cc.b;
+        rcv[q] = cc.c;
       }
-      if (need_in) rin[q] = ldg(a.in_pu, g).a;
+      // phi' of the stage input alone (its u, v are staged)
+      if (need_in) rin[q] = ldg(reinterpret_cast<const float *>(a.in), 3u * g);
       if (!first) {
-        const F2 bpu = ldg(a.b_pu, g);
-        rbp[q] = bpu.a;
-        rbu[q] = bpu.b;
-        rbv[q] = ldg(a.b_v, g);
+        const F3 bb = ldg(a.b, g);
+        rbp[q] = bb.a;
+        rbu[q] = bb.b;
+        rbv[q] = bb.c;
       }
       if (need_acc) {
-        const F2 apu = ldg(a.acc_pu, g);
-        rap[q] = apu.a;
-        rau[q] = apu.b;
-        rav[q] = ldg(a.acc_v, g);
+        const F3 aa = ldg(a.acc, g);
+        rap[q] = aa.a;
+        rau[q] = aa.b;
+        rav[q] = aa.c;
       }
     }
 #pragma unroll
@@ -368,8 +366,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       const double kph = kpg * rsgv;  // d(phi')/dt = d(phiG)/dt / sqrtG
       if (kind == kTend) {
         const uint32_t ti = ((uint32_t)(p - 1) * npv + (j - 1)) * npv + (i - 1);
-        stg(a.o_pu, ti, F2{(float)kpg, (float)ku});
-        stg(a.o_v, ti, (float)kv);
+        stg(a.o, ti, F3{(float)kpg, (float)ku, (float)kv});
       } else {
         // base state Y_n: hi (+ lo in the last stage) in FP64
         double bph, bu, bv;
@@ -390,13 +387,12 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         double ph, u, v;
         if (kind == kRK4S1) {
           ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-          stg(a.acc_pu, g, F2{(float)kph, (float)ku});
-          stg(a.acc_v, g, (float)kv);
+          stg(a.acc, g, F3{(float)kph, (float)ku, (float)kv});
         } else if (kind == kRK4S2 || kind == kRK4S3) {
           ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-          stg(a.acc_pu, g,
-              F2{(float)((double)rap[q] + 2.0 * kph), (float)((double)rau[q] + 2.0 * ku)});
-          stg(a.acc_v, g, (float)((double)rav[q] + 2.0 * kv));
+          stg(a.acc, g,
+              F3{(float)((double)rap[q] + 2.0 * kph), (float)((double)rau[q] + 2.0 * ku),
+                 (float)((double)rav[q] + 2.0 * kv)});
         } else if (kind == kRK4S4) {
           ph = bph + a.c1 * (((double)rap[q] + kph) * (1.0 / 6.0));
           u = bu + a.c1 * (((double)rau[q] + ku) * (1.0 / 6.0));
@@ -413,17 +409,16 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           v = bv + (2.0 / 3.0) * ((double)sV[li] - bv) + a.c1 * kv;
         }
         const float hph = (float)ph, hu = (float)u, hv = (float)v;
-        stg(a.o_pu, g, F2{hph, hu});
-        stg(a.o_v, g, hv);
-        if (a.oc_pu) {  // last stage: keep the rounding residue (Kahan word)
-          stg(a.oc_pu, g, F2{(float)(ph - (double)hph), (float)(u - (double)hu)});
-          stg(a.oc_v, g, (float)(v - (double)hv));
+        stg(a.o, g, F3{hph, hu, hv});
+        if (a.oc) {  // last stage: keep the rounding residue (Kahan word)
+          stg(a.oc, g,
+              F3{(float)(ph - (double)hph), (float)(u - (double)hu), (float)(v - (double)hv)});
         }
         // raw band lines (and their residues in the last stage) for the boundary pass
         if (i <= H + 1 || i >= npv - H || j <= H + 1 || j >= npv - H) {
           const float vals[3] = {hph, hu, hv};
           float lows[3] = {0.0f, 0.0f, 0.0f};
-          if (a.oc_pu) {
+          if (a.oc) {
             lows[0] = (float)(ph - (double)hph);
             lows[1] = (float)(u - (double)hu);
             lows[2] = (float)(v - (double)hv);
@@ -431,7 +426,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           // one store per side the point is near (a corner point is near two)
           auto put = [&](uint32_t sl, int fl) {
             stg(a.band_raw, sl, vals[fl]);
-            if (a.oc_pu) stg(a.band_raw_c, sl, lows[fl]);
+            if (a.oc) stg(a.band_raw_c, sl, lows[fl]);
           };
 #pragma unroll
           for (int fl = 0; fl < 3; ++fl) {
@@ -461,10 +456,8 @@ __global__ __launch_bounds__(kStageThreads) void stage_kernel_f32(MeshF32 m, Sta
 // linear and phi_ref is already continuous across edges).  For the base state
 // (cpu != NULL) values are hi + lo and every written point gets its residue.
 struct BoundaryArgs32 {
-  F2 *pu;                  // level to finish: {phi', u} pairs and v
-  float *v;
-  F2 *cpu;                 // its compensation words (base state only, else NULL)
-  float *cv;
+  F3 *lev;                 // level to finish: {phi', u, v} records
+  F3 *comp;                // its compensation words (base state only, else NULL)
   const float *band_raw, *band_raw_c;  // raw band lines; residues for the base state
 };
 
@@ -472,8 +465,7 @@ template <int DOF>
 __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const BoundaryArgs32 &ba,
                                                   const int t) {
   constexpr int H = DOF - 1;
-  F2 *const pu = ba.pu, *const cpu = ba.cpu;
-  float *const v = ba.v, *const cv = ba.cv;
+  F3 *const lev = ba.lev, *const comp = ba.comp;
   const int npv = m.npv, nh = m.nh;
   const double dx = m.dx;
   const int nghost = 6 * 4 * H * npv, nedge = 12 * npv;
@@ -483,7 +475,7 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
     const uint32_t s0 = band_slot(npv, H, q, side, d, 0, al), s1 = band_slot(npv, H, q, side, d, 1, al),
                    s2 = band_slot(npv, H, q, side, d, 2, al);
     ph = ldg(ba.band_raw, s0); uu = ldg(ba.band_raw, s1); vv = ldg(ba.band_raw, s2);
-    if (cpu) {
+    if (comp) {
       ph += (double)ldg(ba.band_raw_c, s0);
       uu += (double)ldg(ba.band_raw_c, s1);
       vv += (double)ldg(ba.band_raw_c, s2);
@@ -491,12 +483,8 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
   };
   auto store = [&](long long g, double ph, double uu, double vv) {
     const float a0 = (float)ph, a1 = (float)uu, a2 = (float)vv;
-    pu[g] = F2{a0, a1};
-    v[g] = a2;
-    if (cpu) {
-      cpu[g] = F2{(float)(ph - (double)a0), (float)(uu - (double)a1)};
-      cv[g] = (float)(vv - (double)a2);
-    }
+    lev[g] = F3{a0, a1, a2};
+    if (comp) comp[g] = F3{(float)(ph - (double)a0), (float)(uu - (double)a1), (float)(vv - (double)a2)};
   };
   if (t < nghost) {  // one ghost point: (panel, side, depth k, s), s fastest
     const int s = t % npv + 1, r = t / npv, k = r % H + 1, ps = r / H;
@@ -651,30 +639,26 @@ __global__ void ref_gradient_kernel(MeshDev m, const double *phi_ref, F4 *fdE) {
 // (zero in the interior, the halo-and-edge correction elsewhere), u and v,
 // each rounded to FP32 with its residue as the compensation word.
 static __global__ void split_state_kernel(long long np, const double *phi, const double *phi_ref,
-                                   const double *u, const double *v, F2 *hpu, float *hv,
-                                   F2 *cpu, float *cv) {
+                                   const double *u, const double *v, F3 *hi, F3 *lo) {
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
        g += (long long)gridDim.x * blockDim.x) {
     const double d = phi[g] - phi_ref[g];
     const float hd = (float)d, hu = (float)u[g], hvv = (float)v[g];
-    hpu[g] = F2{hd, hu};
-    hv[g] = hvv;
-    cpu[g] = F2{(float)(d - (double)hd), (float)(u[g] - (double)hu)};
-    cv[g] = (float)(v[g] - (double)hvv);
+    hi[g] = F3{hd, hu, hvv};
+    lo[g] = F3{(float)(d - (double)hd), (float)(u[g] - (double)hu), (float)(v[g] - (double)hvv)};
   }
 }
 
 // Reconstructs the FP64 state of an FP32C level: phi = phi_ref + hi + lo,
 // u, v = hi + lo.
-static __global__ void join_state_kernel(long long np, const double *phi_ref, const F2 *hpu,
-                                  const float *hv, const F2 *cpu, const float *cv, double *phi,
-                                  double *u, double *v) {
+static __global__ void join_state_kernel(long long np, const double *phi_ref, const F3 *hi,
+                                         const F3 *lo, double *phi, double *u, double *v) {
   for (long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x; g < np;
        g += (long long)gridDim.x * blockDim.x) {
-    const F2 h = hpu[g], c = cpu[g];
+    const F3 h = hi[g], c = lo[g];
     phi[g] = phi_ref[g] + ((double)h.a + (double)c.a);
     u[g] = (double)h.b + (double)c.b;
-    v[g] = (double)hv[g] + (double)cv[g];
+    v[g] = (double)h.c + (double)c.c;
   }
 }
 

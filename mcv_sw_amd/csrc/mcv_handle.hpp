@@ -98,13 +98,10 @@ struct mcv_handle {
   // compensation words of Y_n, FP32 accumulator and band copies; phi_ref is
   // the projected initial geopotential (FP64).
   bool fp32c = false;
-  // levels, compensation words of Y_n and accumulator: {phi', u} pairs and v
-  mcv::F2 *l32pu[3] = {};
-  float *l32v[3] = {};
-  mcv::F2 *c32pu = nullptr;
-  float *c32v = nullptr;
-  mcv::F2 *a32pu = nullptr;
-  float *a32v = nullptr;
+  // levels, compensation words of Y_n and accumulator: {phi', u, v} records
+  mcv::F3 *l32[3] = {};
+  mcv::F3 *c32 = nullptr;
+  mcv::F3 *a32 = nullptr;
   float *band32 = nullptr, *band32c = nullptr;  // FP32 band lines and residues
   double *join[3] = {};  // FP64 reconstruction of Y_n for read-back and integrals
   mcv::MeshF32 mesh32{};

@@ -52,7 +52,7 @@ size_t footprint_bytes(const mcv_config *c) {
   b += d(nband) + 2 * d(np) + d(4 * diag_blocks) + d(2) + 6 * d(np);
   if (fp32c)  // join, phi_ref, phi_ref*sqrtG, FP32 {reference gradient, Coriolis, 0},
               // levels, compensation, accumulator, bands
-    b += 3 * d(np) + d(np) + d(np) + 4 * f(np) + 9 * f(np) + 3 * f(np) + 3 * f(np) +
+    b += 3 * d(np) + d(np) + d(np) + 4 * f(np) + 3 * f(3 * np) + f(3 * np) + f(3 * np) +
          2 * f(nband);
   return b;
 }
@@ -74,14 +74,12 @@ BoundaryArgs boundary_args64(const mcv_handle *h, int level) {
 
 StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1) {
   StageArgs32 a{};
-  a.in_pu = h->l32pu[in]; a.in_v = h->l32v[in];
-  a.b_pu = h->l32pu[0]; a.b_v = h->l32v[0];
-  a.c_pu = h->c32pu; a.c_v = h->c32v;
-  a.acc_pu = h->a32pu; a.acc_v = h->a32v;
-  a.o_pu = h->l32pu[out]; a.o_v = h->l32v[out];
-  if (out == 0) {  // the step's last stage writes Y_n+1 with its compensation
-    a.oc_pu = h->c32pu; a.oc_v = h->c32v;
-  }
+  a.in = h->l32[in];
+  a.b = h->l32[0];
+  a.c = h->c32;
+  a.acc = h->a32;
+  a.o = h->l32[out];
+  if (out == 0) a.oc = h->c32;  // the step's last stage writes Y_n+1 with its compensation
   a.band_raw = h->band32;
   a.band_raw_c = h->band32c;
   a.c1 = c1;
@@ -91,8 +89,8 @@ StageArgs32 stage_args32(const mcv_handle *h, int in, int out, double c1) {
 BoundaryArgs32 boundary_args32(const mcv_handle *h, int level) {
   const bool base = level == 0;  // Y_n carries its compensation words
   BoundaryArgs32 b{};
-  b.pu = h->l32pu[level]; b.v = h->l32v[level];
-  if (base) { b.cpu = h->c32pu; b.cv = h->c32v; }
+  b.lev = h->l32[level];
+  if (base) b.comp = h->c32;
   b.band_raw = h->band32;
   b.band_raw_c = h->band32c;
   return b;
@@ -357,8 +355,7 @@ int materialize(mcv_handle *h, const double **out) {
     return MCV_OK;
   }
   hipLaunchKernelGGL(join_state_kernel, dim3(1024), dim3(256), 0, h->stream, h->np, h->phi_ref,
-                     h->l32pu[0], h->l32v[0], h->c32pu, h->c32v, h->join[0], h->join[1],
-                     h->join[2]);
+                     h->l32[0], h->c32, h->join[0], h->join[1], h->join[2]);
   HIP_TRY(hipGetLastError());
   for (int f = 0; f < 3; ++f) out[f] = h->join[f];
   return MCV_OK;
@@ -441,11 +438,8 @@ int setup32(mcv_handle *h) {
                        h->phi_ref, fdE);
   HIP_TRY(hipGetLastError());
   for (int l = 0; l < 3; ++l)
-    if ((rc = dev_alloc_t(h, &h->l32pu[l], np)) || (rc = dev_alloc_t(h, &h->l32v[l], np)))
-      return rc;
-  if ((rc = dev_alloc_t(h, &h->c32pu, np)) || (rc = dev_alloc_t(h, &h->c32v, np)) ||
-      (rc = dev_alloc_t(h, &h->a32pu, np)) || (rc = dev_alloc_t(h, &h->a32v, np)))
-    return rc;
+    if ((rc = dev_alloc_t(h, &h->l32[l], np))) return rc;
+  if ((rc = dev_alloc_t(h, &h->c32, np)) || (rc = dev_alloc_t(h, &h->a32, np))) return rc;
   if ((rc = dev_alloc_t(h, &h->band32, nband)) || (rc = dev_alloc_t(h, &h->band32c, nband)))
     return rc;
   double *pgref;
@@ -454,15 +448,11 @@ int setup32(mcv_handle *h) {
                      pgref);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(split_state_kernel, dim3(1024), dim3(256), 0, h->stream, np, h->join[0],
-                     h->phi_ref, h->join[1], h->join[2], h->l32pu[0], h->l32v[0], h->c32pu,
-                     h->c32v);
+                     h->phi_ref, h->join[1], h->join[2], h->l32[0], h->c32);
   HIP_TRY(hipGetLastError());
-  for (int l = 1; l < 3; ++l) {
-    HIP_TRY(hipMemcpyAsync(h->l32pu[l], h->l32pu[0], np * sizeof(F2), hipMemcpyDeviceToDevice,
+  for (int l = 1; l < 3; ++l)
+    HIP_TRY(hipMemcpyAsync(h->l32[l], h->l32[0], np * sizeof(F3), hipMemcpyDeviceToDevice,
                            h->stream));
-    HIP_TRY(hipMemcpyAsync(h->l32v[l], h->l32v[0], np * sizeof(float), hipMemcpyDeviceToDevice,
-                           h->stream));
-  }
   MeshF32 &m = h->mesh32;
   const MeshDev &d = h->mesh;
   m.dof = d.dof;
