@@ -48,8 +48,8 @@ struct StageArgs32 {
   F3 *acc;                                 // RK4 accumulator
   F3 *o;                                   // stage output (raw interior)
   F3 *oc;                                  // compensation of the output (last stage only)
-  float *band_raw;                         // raw band lines (hi), see band_slot()
-  float *band_raw_c;                       // raw band lines (lo), last stage only
+  F3 *band_raw;                            // raw band lines (hi), see band_point()
+  F3 *band_raw_c;                          // raw band lines (lo), last stage only
   double c1;                               // stage increment coefficient (x dt)
 };
 
@@ -423,18 +423,15 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
             lows[1] = (float)(u - (double)hu);
             lows[2] = (float)(v - (double)hv);
           }
-          // one store per side the point is near (a corner point is near two)
-          auto put = [&](uint32_t sl, int fl) {
-            stg(a.band_raw, sl, vals[fl]);
-            if (a.oc) stg(a.band_raw_c, sl, lows[fl]);
+          // one record per side the point is near (a corner point is near two)
+          auto put = [&](uint32_t bp) {
+            stg(a.band_raw, bp, F3{vals[0], vals[1], vals[2]});
+            if (a.oc) stg(a.band_raw_c, bp, F3{lows[0], lows[1], lows[2]});
           };
-#pragma unroll
-          for (int fl = 0; fl < 3; ++fl) {
-            if (i <= H + 1) put(band_slot(npv, H, p, 0, i, fl, j), fl);
-            if (i >= npv - H) put(band_slot(npv, H, p, 1, npv + 1 - i, fl, j), fl);
-            if (j <= H + 1) put(band_slot(npv, H, p, 2, j, fl, i), fl);
-            if (j >= npv - H) put(band_slot(npv, H, p, 3, npv + 1 - j, fl, i), fl);
-          }
+          if (i <= H + 1) put(band_point(npv, H, p, 0, i, j));
+          if (i >= npv - H) put(band_point(npv, H, p, 1, npv + 1 - i, j));
+          if (j <= H + 1) put(band_point(npv, H, p, 2, j, i));
+          if (j >= npv - H) put(band_point(npv, H, p, 3, npv + 1 - j, i));
         }
       }
     }
@@ -458,7 +455,7 @@ __global__ __launch_bounds__(kStageThreads) void stage_kernel_f32(MeshF32 m, Sta
 struct BoundaryArgs32 {
   F3 *lev;                 // level to finish: {phi', u, v} records
   F3 *comp;                // its compensation words (base state only, else NULL)
-  const float *band_raw, *band_raw_c;  // raw band lines; residues for the base state
+  const F3 *band_raw, *band_raw_c;  // raw band records; residues for the base state
 };
 
 template <int DOF>
@@ -472,13 +469,14 @@ __device__ __forceinline__ void boundary_item_f32(const MeshDev &m, const Bounda
   // raw value of band point (panel q, side, depth d, along index al): hi, plus
   // lo for the base state
   auto raw = [&](int q, int side, int d, int al, double &ph, double &uu, double &vv) {
-    const uint32_t s0 = band_slot(npv, H, q, side, d, 0, al), s1 = band_slot(npv, H, q, side, d, 1, al),
-                   s2 = band_slot(npv, H, q, side, d, 2, al);
-    ph = ldg(ba.band_raw, s0); uu = ldg(ba.band_raw, s1); vv = ldg(ba.band_raw, s2);
+    const uint32_t bp = band_point(npv, H, q, side, d, al);
+    const F3 r = ldg(ba.band_raw, bp);
+    ph = r.a; uu = r.b; vv = r.c;
     if (comp) {
-      ph += (double)ldg(ba.band_raw_c, s0);
-      uu += (double)ldg(ba.band_raw_c, s1);
-      vv += (double)ldg(ba.band_raw_c, s2);
+      const F3 c = ldg(ba.band_raw_c, bp);
+      ph += (double)c.a;
+      uu += (double)c.b;
+      vv += (double)c.c;
     }
   };
   auto store = [&](long long g, double ph, double uu, double vv) {

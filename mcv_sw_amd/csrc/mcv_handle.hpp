@@ -102,7 +102,7 @@ struct mcv_handle {
   mcv::F3 *l32[3] = {};
   mcv::F3 *c32 = nullptr;
   mcv::F3 *a32 = nullptr;
-  float *band32 = nullptr, *band32c = nullptr;  // FP32 band lines and residues
+  mcv::F3 *band32 = nullptr, *band32c = nullptr;  // FP32 band records and residues
   double *join[3] = {};  // FP64 reconstruction of Y_n for read-back and integrals
   mcv::MeshF32 mesh32{};
 };

@@ -440,7 +440,8 @@ int setup32(mcv_handle *h) {
   for (int l = 0; l < 3; ++l)
     if ((rc = dev_alloc_t(h, &h->l32[l], np))) return rc;
   if ((rc = dev_alloc_t(h, &h->c32, np)) || (rc = dev_alloc_t(h, &h->a32, np))) return rc;
-  if ((rc = dev_alloc_t(h, &h->band32, nband)) || (rc = dev_alloc_t(h, &h->band32c, nband)))
+  if ((rc = dev_alloc_t(h, &h->band32, nband / 3)) ||
+      (rc = dev_alloc_t(h, &h->band32c, nband / 3)))
     return rc;
   double *pgref;
   if ((rc = dev_alloc_t(h, &pgref, np))) return rc;
