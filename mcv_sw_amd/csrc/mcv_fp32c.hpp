@@ -409,24 +409,17 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
           v = bv + (2.0 / 3.0) * ((double)sV[li] - bv) + a.c1 * kv;
         }
         const float hph = (float)ph, hu = (float)u, hv = (float)v;
-        stg(a.o, g, F3{hph, hu, hv});
-        if (a.oc) {  // last stage: keep the rounding residue (Kahan word)
-          stg(a.oc, g,
-              F3{(float)(ph - (double)hph), (float)(u - (double)hu), (float)(v - (double)hv)});
-        }
-        // raw band lines (and their residues in the last stage) for the boundary pass
+        const F3 hi{hph, hu, hv};
+        // the rounding residue (Kahan word), kept by the last stage only
+        const F3 lo{(float)(ph - (double)hph), (float)(u - (double)hu), (float)(v - (double)hv)};
+        stg(a.o, g, hi);
+        if (a.oc) stg(a.oc, g, lo);
+        // raw band records (and their residues in the last stage) for the boundary pass
         if (i <= H + 1 || i >= npv - H || j <= H + 1 || j >= npv - H) {
-          const float vals[3] = {hph, hu, hv};
-          float lows[3] = {0.0f, 0.0f, 0.0f};
-          if (a.oc) {
-            lows[0] = (float)(ph - (double)hph);
-            lows[1] = (float)(u - (double)hu);
-            lows[2] = (float)(v - (double)hv);
-          }
           // one record per side the point is near (a corner point is near two)
           auto put = [&](uint32_t bp) {
-            stg(a.band_raw, bp, F3{vals[0], vals[1], vals[2]});
-            if (a.oc) stg(a.band_raw_c, bp, F3{lows[0], lows[1], lows[2]});
+            stg(a.band_raw, bp, hi);
+            if (a.oc) stg(a.band_raw_c, bp, lo);
           };
           if (i <= H + 1) put(band_point(npv, H, p, 0, i, j));
           if (i >= npv - H) put(band_point(npv, H, p, 1, npv + 1 - i, j));
