@@ -177,6 +177,27 @@ __host__ __device__ inline void edge_ij(int npv, int side, int s, int &i, int &j
   else { i = s; j = npv; }
 }
 
+// Index arithmetic at full rate.  The 32-bit integer multiply and the
+// multiply-high behind a division by a constant issue at a quarter of the
+// VALU rate on gfx950; the 24-bit multiply issues at the full rate.
+// mul24: a * b for a, b < 2^24 whose product is below 2^32.
+__device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) { return __umul24(a, b); }
+// div_small<D>(t) = t / D for 0 <= t < 2048 and D <= 1024: with
+// M = ceil(2^21 / D) = (2^21 + e) / D, 0 <= e < D,
+// t * M / 2^21 = t / D + t * e / (D * 2^21), whose floor is floor(t / D)
+// while t * e < 2^21 (t < 2048 <= 2^21 / D); t * M < 2048 * (2^21 / 3 + 1)
+// stays below 2^32 (D = 3 is the smallest divisor that is not a power of 2).
+template <int D>
+__device__ __forceinline__ int div_small(int t) {
+  static_assert(D >= 1 && D <= 1024, "divisor range");
+  if constexpr ((D & (D - 1)) == 0) {
+    return (int)((uint32_t)t / (uint32_t)D);
+  } else {
+    constexpr uint32_t M = ((1u << 21) + D - 1) / D;
+    return (int)(mul24((uint32_t)t, M) >> 21);
+  }
+}
+
 // Element idx of a device array addressed as a uniform base plus a 32-bit
 // byte offset, which the global load/store instructions take directly (no
 // 64-bit address arithmetic per access).  The library keeps every array that
@@ -191,6 +212,16 @@ __device__ __forceinline__ void stg(T *base, uint32_t idx, T v) {
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (size_t)(idx * (uint32_t)sizeof(T))) = v;
 }
 
+// The same at a byte offset formed by the caller (for 12-byte records, whose
+// offset idx * 12 would otherwise take a quarter-rate multiply).
+template <typename T>
+__device__ __forceinline__ T ldgb(const T *base, uint32_t byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (size_t)byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void stgb(T *base, uint32_t byte_off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (size_t)byte_off) = v;
+}
 
 // Linear index of point (i, j) of panel p (1-based, halo-relative i, j).
 __host__ __device__ inline long long pidx(int nh, int H, int p, int i, int j) {

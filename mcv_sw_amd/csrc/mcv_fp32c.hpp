@@ -121,6 +121,10 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
   const uint32_t pbase = (uint32_t)(p - 1) * (uint32_t)(nh * nh);
   const uint32_t mbase = (uint32_t)(p - 1) * (uint32_t)m.metric_pstride;
   const D2 *const mpk = m.mpack + 2 * (size_t)mbase;  // this panel's packed metric terms
+  // byte offset of point (row, col) of this panel in the 12-byte record arrays,
+  // from full-rate 24-bit multiplies (row, col < nh <= 6148)
+  const uint32_t pb12 = 12u * pbase, nh12 = 12u * (uint32_t)nh;
+  auto rec = [&](int row, int col) { return pb12 + mul24(row, nh12) + mul24(col, 12u); };
 
   // loads of all a thread's points first, then the derived values
   constexpr int kP0 = (TX * TY + kStageThreads - 1) / kStageThreads;
@@ -132,13 +136,13 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
 #pragma unroll
     for (int q = 0; q < kP0; ++q) {
       const int t = t0 + q * blockDim.x;
-      const int ly = t / TX, lx = t - ly * TX;
+      const int ly = div_small<TX>(t), lx = t - ly * TX;
       const bool inx = lx >= H && lx < H + nx, iny = ly >= H && ly < H + ny;
       live[q] = t < TX * TY && lx < lw && ly < lh && (inx || iny);
       if (!live[q]) continue;
-      const uint32_t off = (uint32_t)((j0 - 1 + ly) * nh + (i0 - 1 + lx));
+      const uint32_t off = mul24(j0 - 1 + ly, nh) + (uint32_t)(i0 - 1 + lx);
       const uint32_t g = pbase + off;
-      rs[q] = ldg(a.in, g);
+      rs[q] = ldgb(a.in, rec(j0 - 1 + ly, i0 - 1 + lx));
       rpg[q] = ldg(m.pgref, g);
       const D2 ga = ldg(mpk, 2 * off), gb = ldg(mpk, 2 * off + 1);
       rsg[q] = ga.a;
@@ -150,7 +154,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
     for (int q = 0; q < kP0; ++q) {
       if (!live[q]) continue;
       const int li = t0 + q * blockDim.x;  // = ly * TX + lx
-      const int ly = li / TX, lx = li - ly * TX;
+      const int ly = div_small<TX>(li), lx = li - ly * TX;
       const float dp = rs[q].a, u = rs[q].b, v = rs[q].c;
       // metric products in FP64 (the FP32 inputs are exact in FP64), rounded once
       const double ud = u, vd = v;
@@ -170,13 +174,14 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         const int ox = lx - H, oy = ly - H;
         lds.sSG[TL::core_at(lx, ly)] = rsg[q];
         const C phi = (C)((double)pg / rsg[q]);  // for the wave speeds only
-        if (ox % H == 0) {
+        const int cx = div_small<H>(ox), cy = div_small<H>(oy);
+        if (ox == cx * H) {
           const C sq = sqrt((C)g11 * phi);
-          lamx[(ox / H) * NY + oy] = fmax(fabs(cu - sq), fabs(cu + sq));
+          lamx[cx * NY + oy] = fmax(fabs(cu - sq), fabs(cu + sq));
         }
-        if (oy % H == 0) {
+        if (oy == cy * H) {
           const C sq = sqrt((C)g22 * phi);
-          lamy[(oy / H) * NX + ox] = fmax(fabs(cv - sq), fabs(cv + sq));
+          lamy[cy * NX + ox] = fmax(fabs(cv - sq), fabs(cv + sq));
         }
       }
     }
@@ -188,8 +193,8 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
     const bool isx = t < kTasksX;
     const int tt = isx ? t : t - kTasksX;
     int mb, o;
-    if (isx) { mb = tt / NY; o = tt - mb * NY; }
-    else { mb = tt / NX; o = tt - mb * NX; }
+    if (isx) { mb = div_small<NY>(tt); o = tt - mb * NY; }
+    else { mb = div_small<NX>(tt); o = tt - mb * NX; }
     if (isx ? (mb > ncx || o >= ny) : (mb > ncy || o >= nx)) continue;
     const int st = isx ? 1 : TX;
     const int b = isx ? (o + H) * TX + (H + mb * H) : (H + mb * H) * TX + (o + H);
@@ -264,11 +269,12 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       int ox = 0, oy = 0;
       bool ok = false;
       if (t < kSlots) {
-        const int cls = t / kCells, cell = t - cls * kCells;
-        const int cyl = cell / CX, cxl = cell - cyl * CX;
+        const int cls = div_small<kCells>(t), cell = t - cls * kCells;
+        const int cyl = div_small<CX>(cell), cxl = cell - cyl * CX;
+        const int cly = div_small<H>(cls);
         ok = cxl < ncx && cyl < ncy;
-        ox = cxl * H + cls % H;
-        oy = cyl * H + cls / H;
+        ox = cxl * H + (cls - cly * H);
+        oy = cyl * H + cly;
       } else if (t < kSlots + NY) {
         oy = t - kSlots;
         ox = ncx * H;
@@ -282,29 +288,30 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       rox[q] = ox;
       roy[q] = oy;
       if (!ok) continue;
-      const uint32_t off = (uint32_t)((j0 + oy + H - 1) * nh + (i0 + ox + H - 1));
+      const uint32_t off = mul24(j0 + oy + H - 1, nh) + (uint32_t)(i0 + ox + H - 1);
       const uint32_t g = pbase + off;
+      const uint32_t gb = rec(j0 + oy + H - 1, i0 + ox + H - 1);
       const F4 fde = ldg(m.fdE, g);
       rdx_[q] = fde.a;
       rdy_[q] = fde.b;
       rf[q] = fde.c;
       if (kind == kTend) continue;
       if (last) {
-        const F3 cc = ldg(a.c, g);
+        const F3 cc = ldgb(a.c, gb);
         rcp[q] = cc.a;
         rcu[q] = cc.b;
         rcv[q] = cc.c;
       }
       // phi' of the stage input alone (its u, v are staged)
-      if (need_in) rin[q] = ldg(reinterpret_cast<const float *>(a.in), 3u * g);
+      if (need_in) rin[q] = ldgb(reinterpret_cast<const float *>(a.in), gb);
       if (!first) {
-        const F3 bb = ldg(a.b, g);
+        const F3 bb = ldgb(a.b, gb);
         rbp[q] = bb.a;
         rbu[q] = bb.b;
         rbv[q] = bb.c;
       }
       if (need_acc) {
-        const F3 aa = ldg(a.acc, g);
+        const F3 aa = ldgb(a.acc, gb);
         rap[q] = aa.a;
         rau[q] = aa.b;
         rav[q] = aa.c;
@@ -317,7 +324,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
       const int lx = ox + H, ly = oy + H, li = ly * TX + lx;
       C fx, dvx, vx, fy, dvy, uy;
       {
-        const int mb = ox / H, pos = ox - mb * H;
+        const int mb = div_small<H>(ox), pos = ox - mb * H;
         const int r0 = mb * NY + oy;
         if (pos == 0) {
           fx = rxF[r0]; dvx = rxD[r0]; vx = rxV[r0];
@@ -335,7 +342,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         }
       }
       {
-        const int mb = oy / H, pos = oy - mb * H;
+        const int mb = div_small<H>(oy), pos = oy - mb * H;
         const int r0 = mb * NX + ox;
         if (pos == 0) {
           fy = ryF[r0]; dvy = ryD[r0]; uy = ryU[r0];
@@ -354,7 +361,7 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         }
       }
       const int i = i0 + ox, j = j0 + oy;
-      const uint32_t g = pbase + (uint32_t)((j + H - 1) * nh + (i + H - 1));
+      const uint32_t gb = rec(j + H - 1, i + H - 1);
       // tendencies assembled in FP64 (spatial_operators_mod.F90:127-131)
       const double sg = lds.sSG[TL::core_at(lx, ly)], rsgv = 1.0 / sg;
       const double fc = (double)rf[q];
@@ -387,11 +394,11 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         double ph, u, v;
         if (kind == kRK4S1) {
           ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-          stg(a.acc, g, F3{(float)kph, (float)ku, (float)kv});
+          stgb(a.acc, gb, F3{(float)kph, (float)ku, (float)kv});
         } else if (kind == kRK4S2 || kind == kRK4S3) {
           ph = bph + a.c1 * kph; u = bu + a.c1 * ku; v = bv + a.c1 * kv;
-          stg(a.acc, g,
-              F3{(float)((double)rap[q] + 2.0 * kph), (float)((double)rau[q] + 2.0 * ku),
+          stgb(a.acc, gb,
+               F3{(float)((double)rap[q] + 2.0 * kph), (float)((double)rau[q] + 2.0 * ku),
                  (float)((double)rav[q] + 2.0 * kv)});
         } else if (kind == kRK4S4) {
           ph = bph + a.c1 * (((double)rap[q] + kph) * (1.0 / 6.0));
@@ -412,8 +419,8 @@ __device__ __forceinline__ void stage_tile_f32(const MeshF32 &m, const StageArgs
         const F3 hi{hph, hu, hv};
         // the rounding residue (Kahan word), kept by the last stage only
         const F3 lo{(float)(ph - (double)hph), (float)(u - (double)hu), (float)(v - (double)hv)};
-        stg(a.o, g, hi);
-        if (a.oc) stg(a.oc, g, lo);
+        stgb(a.o, gb, hi);
+        if (a.oc) stgb(a.oc, gb, lo);
         // raw band records (and their residues in the last stage) for the boundary pass
         if (i <= H + 1 || i >= npv - H || j <= H + 1 || j >= npv - H) {
           // one record per side the point is near (a corner point is near two)

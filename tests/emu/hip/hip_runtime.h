@@ -65,6 +65,8 @@ inline void __syncthreads() { g_emu_block_barrier->arrive_and_wait(); }
 inline void __syncthreads() {}
 #endif
 inline void __threadfence() {}
+// the 24-bit multiply: low 32 bits of the product of the low 24 bits
+inline unsigned __umul24(unsigned a, unsigned b) { return (a & 0xffffffu) * (b & 0xffffffu); }
 inline unsigned atomicAdd(unsigned *p, unsigned v) {
   const unsigned o = *p;
   *p = o + v;

@@ -71,3 +71,21 @@ def test_fp64_stage_kernels_within_80_vgprs(kernels):
             if re.match(r"_ZN3mcv12stage_kernelILi4ELi\dELi(6ELi4|5ELi5)E", k)}
     assert len(fp64) == 8 * 2  # stage kinds (tendency + 7) x tile shapes
     assert all(v["vgpr"] <= 80 and v["occupancy"] >= 5 for v in fp64.values()), fp64
+
+
+def test_div_small_exact_on_its_domain():
+    """div_small<D>(t) (mcv_device.hpp) forms t / D as (t * ceil(2^21 / D)) >> 21
+    with one 24-bit multiply: exhaustively, for every divisor 1..1024 that is
+    not a power of two and every t < 2048, the quotient is exact and the
+    product stays below 2^32."""
+    import numpy as np
+
+    t = np.arange(2048, dtype=np.uint64)
+    for d in range(3, 1025):
+        if d & (d - 1) == 0:
+            continue
+        m = ((1 << 21) + d - 1) // d
+        assert m < (1 << 24)
+        prod = t * np.uint64(m)
+        assert int(prod.max()) < (1 << 32), d
+        assert np.array_equal(prod >> np.uint64(21), t // np.uint64(d)), d
